@@ -1,0 +1,503 @@
+// fx_csr.hip -- CSRx matrices in HBM: creation, SpMV (MAT::Apply),
+// residual (MAT::Residual), diagonal (MAT::GetDiag) and transpose
+// (MAT::TransInPlace), src/MAT.cxx.
+#include <cstring>
+
+#include "fx_spmv.hpp"
+
+namespace fx {
+
+namespace {
+
+// Row-block table (see fx_csr_s).
+std::vector<uint32_t> build_row_blocks(uint32_t nrow, const uint32_t* rp, uint32_t* max_row) {
+  std::vector<uint32_t> blk;
+  blk.reserve(nrow / 128 + 2);
+  blk.push_back(0);
+  uint32_t r = 0, mx = 0;
+  while (r < nrow) {
+    const uint32_t start = r;
+    const uint32_t len = rp[r + 1] - rp[r];
+    if (len > mx) mx = len;
+    if (len > kBlockNnz) {
+      ++r;
+    } else {
+      uint32_t nz = 0;
+      while (r < nrow && r - start < kBlockRows) {
+        const uint32_t l = rp[r + 1] - rp[r];
+        if (l > mx) mx = l;
+        if (nz + l > kBlockNnz) break;
+        nz += l;
+        ++r;
+      }
+    }
+    blk.push_back(r);
+  }
+  *max_row = mx;
+  return blk;
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+// diagPtr as MAT::FormDiagPtr (src/MAT.cxx:917-928): first k with col == row;
+// rows without a diagonal keep 0 (the reference's freshly resized vector).
+__global__ void k_form_diagptr(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                               uint32_t* __restrict__ dp, uint32_t nrow) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= nrow) return;
+  uint32_t d = 0;
+  for (uint32_t k = rp[j]; k < rp[j + 1]; ++k) {
+    if (ci[k] == j) {
+      d = k;
+      break;
+    }
+  }
+  dp[j] = d;
+}
+
+// v_j = values[diagPtr[j]] (or 1 for a structure-only matrix), src/MAT.cxx:256-267
+template <class T>
+__global__ void k_get_diag(const double* __restrict__ va, const uint32_t* __restrict__ dp,
+                           T* __restrict__ d, uint32_t n) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= n) return;
+  d[j] = va ? (T)va[dp[j]] : (T)1;
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, uint32_t n) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) d[i] = (float)s[i];
+}
+
+// max |x_i| into *out (one reduction)
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_norminf(const T* __restrict__ x, uint32_t n,
+                                                      double* partials, unsigned* counter,
+                                                      double* out) {
+  double acc[1] = {0.0};
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    acc[0] = fmax(acc[0], fabs((double)x[i]));
+  if (grid_reduce<1, 1u>(acc, partials, counter) && threadIdx.x == 0) *out = acc[0];
+}
+
+// r = b if ||x||_inf < CLOSE_ZERO, else the row chain b - A x
+// (MAT::Residual, src/MAT.cxx:459-469).
+template <class VT>
+struct EpiResidual {
+  VT* __restrict__ r;
+  const VT* __restrict__ b;
+  const double* __restrict__ xinf;
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&)[1]) const {
+    r[i] = (*xinf < kCloseZero) ? b[i] : (VT)v;
+  }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+// ---- transpose (MAT::TransInPlace, src/MAT.cxx:479-538) -------------------
+// Counting sort by column.  Placement within a transposed row is first done
+// with atomics (any order), then each row's entries are ranked by their
+// source position k, which restores the reference's order (source rows
+// ascending, ties by position) deterministically.
+
+__global__ void k_col_count(const uint32_t* __restrict__ ci, uint32_t nnz, uint32_t* cnt) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < nnz; k += stride)
+    atomicAdd(&cnt[ci[k]], 1u);
+}
+
+// Exclusive scan of cnt[0..m) into out[0..m], single workgroup, chunked.
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ cnt, uint32_t m,
+                                               uint32_t* __restrict__ out) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < m; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < m ? cnt[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < m) out[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = carry;
+}
+
+__global__ void k_row_ids(const uint32_t* __restrict__ rp, uint32_t nrow, uint32_t* __restrict__ rid) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= nrow) return;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) rid[k] = i;
+}
+
+__global__ void k_scatter(const uint32_t* __restrict__ ci, uint32_t nnz, uint32_t* cursor,
+                          uint32_t* __restrict__ perm) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < nnz; k += stride) {
+    const uint32_t pos = atomicAdd(&cursor[ci[k]], 1u);
+    perm[pos] = k;
+  }
+}
+
+// One wave per transposed row: entry e's final slot is the number of entries
+// of the row with a smaller source position.
+__global__ __launch_bounds__(kThreads) void k_rank_rows(const uint32_t* __restrict__ rpT, uint32_t m,
+                                                        const uint32_t* __restrict__ perm,
+                                                        uint32_t* __restrict__ sorted) {
+  const uint32_t row = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (row >= m) return;
+  const uint32_t s = rpT[row], e = rpT[row + 1], L = e - s;
+  for (uint32_t a = lane; a < L; a += 64) {
+    const uint32_t key = perm[s + a];
+    uint32_t rank = 0;
+    for (uint32_t f = 0; f < L; ++f) rank += perm[s + f] < key ? 1u : 0u;
+    sorted[s + rank] = key;
+  }
+}
+
+__global__ void k_gather_t(const uint32_t* __restrict__ sorted, uint32_t nnz,
+                           const uint32_t* __restrict__ rid, const double* __restrict__ va,
+                           uint32_t* __restrict__ ciT, double* __restrict__ vaT) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < nnz; p += stride) {
+    const uint32_t k = sorted[p];
+    ciT[p] = rid[k];
+    if (va) vaT[p] = va[k];
+  }
+}
+
+int csr_alloc(fx_csr A) {
+  const size_t n1 = (size_t)A->nrow + 1, nz = A->nnz ? A->nnz : 1, nd = A->nrow ? A->nrow : 1;
+  FX_HIP(hipMalloc(&A->rp, sizeof(uint32_t) * n1));
+  FX_HIP(hipMalloc(&A->ci, sizeof(uint32_t) * nz));
+  FX_HIP(hipMalloc(&A->dp, sizeof(uint32_t) * nd));
+  FX_HIP(hipMalloc(&A->va, sizeof(double) * nz));
+  return 0;
+}
+
+int csr_set_blocks(fx_csr A, const uint32_t* host_rp) {
+  uint32_t mx = 0;
+  std::vector<uint32_t> blk = build_row_blocks(A->nrow, host_rp, &mx);
+  A->nblk = (uint32_t)blk.size() - 1;
+  A->max_row = mx;
+  FX_HIP(hipMalloc(&A->blk, sizeof(uint32_t) * blk.size()));
+  FX_HIP(hipMemcpyAsync(A->blk, blk.data(), sizeof(uint32_t) * blk.size(), hipMemcpyHostToDevice,
+                        A->ctx->stream));
+  FX_HIP(hipStreamSynchronize(A->ctx->stream));
+  return 0;
+}
+
+void csr_free(fx_csr A) {
+  if (!A) return;
+  if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+  (void)hipFree(A->rp);
+  (void)hipFree(A->ci);
+  (void)hipFree(A->dp);
+  (void)hipFree(A->va);
+  (void)hipFree(A->va32);
+  (void)hipFree(A->blk);
+  delete A;
+}
+
+}  // namespace
+
+int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype) {
+  if (A->nrow == 0) return 0;
+  if (dtype == FX_F64)
+    return launch_spmv<double, double, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<double>{(const double*)x},
+                                                      BNone{}, EpiStore<double>{(double*)y}, nullptr,
+                                                      nullptr);
+  FX_CHECK(A->va32 || !A->has_values, FX_ERROR_INPUT_PAR,
+           "f32 SpMV needs the f32 value copy (fx_csr_make_f32)");
+  return launch_spmv<float, float, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<float>{(const float*)x},
+                                                  BNone{}, EpiStore<float>{(float*)y}, nullptr,
+                                                  nullptr);
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_csr_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, uint32_t nnz, const uint32_t* rowPtr,
+                  const uint32_t* colInd, const double* values, const uint32_t* diagPtr,
+                  fx_csr* out) {
+  clear_error();
+  FX_CHECK(ctx && out, FX_ERROR_INPUT_PAR, "null argument");
+  fx_csr A = new fx_csr_s();
+  A->ctx = ctx;
+  if (nrow == 0 || mcol == 0 || nnz == 0) {  // MAT::Empty, src/MAT.cxx:26-29
+    A->nrow = A->mcol = A->nnz = 0;
+    *out = A;
+    return 0;
+  }
+  auto fail = [&](int code) {
+    csr_free(A);
+    return code;
+  };
+  if (!rowPtr || !colInd) {
+    set_error("null rowPtr / colInd");
+    return fail(FX_ERROR_INPUT_PAR);
+  }
+  // Structure validation (the device kernels index with these arrays).
+  if (rowPtr[0] != 0 || rowPtr[nrow] != nnz) {
+    set_error("rowPtr[0] = %u, rowPtr[nrow] = %u, nnz = %u", rowPtr[0], rowPtr[nrow], nnz);
+    return fail(FX_ERROR_MAT_DATA);
+  }
+  for (uint32_t j = 0; j < nrow; ++j) {
+    if (rowPtr[j] > rowPtr[j + 1]) {
+      set_error("rowPtr decreases at row %u", j);
+      return fail(FX_ERROR_MAT_DATA);
+    }
+  }
+  for (uint32_t k = 0; k < nnz; ++k) {
+    if (colInd[k] >= mcol) {
+      set_error("colInd[%u] = %u >= mcol = %u", k, colInd[k], mcol);
+      return fail(FX_ERROR_MAT_DATA);
+    }
+  }
+  const uint32_t ndiag = nrow < mcol ? nrow : mcol;
+  if (diagPtr) {
+    for (uint32_t j = 0; j < ndiag; ++j) {
+      if (diagPtr[j] >= nnz) {
+        set_error("diagPtr[%u] = %u >= nnz", j, diagPtr[j]);
+        return fail(FX_ERROR_MAT_DATA);
+      }
+    }
+  }
+  A->nrow = nrow;
+  A->mcol = mcol;
+  A->nnz = nnz;
+  A->has_values = values != nullptr;
+  int rc = csr_alloc(A);
+  if (rc) return fail(rc);
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(A->rp, rowPtr, sizeof(uint32_t) * (nrow + 1), hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(A->ci, colInd, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st) ||
+      (values &&
+       hipMemcpyAsync(A->va, values, sizeof(double) * nnz, hipMemcpyHostToDevice, st))) {
+    set_error("upload failed");
+    return fail(FX_ERROR_DEVICE);
+  }
+  if (diagPtr) {
+    if (hipMemsetAsync(A->dp, 0, sizeof(uint32_t) * nrow, st) ||
+        hipMemcpyAsync(A->dp, diagPtr, sizeof(uint32_t) * ndiag, hipMemcpyHostToDevice, st)) {
+      set_error("upload failed");
+      return fail(FX_ERROR_DEVICE);
+    }
+  } else {
+    hipLaunchKernelGGL(k_form_diagptr, dim3((nrow + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       st, A->rp, A->ci, A->dp, nrow);
+  }
+  rc = csr_set_blocks(A, rowPtr);
+  if (rc) return fail(rc);
+  *out = A;
+  return 0;
+}
+
+int fx_csr_destroy(fx_csr A) {
+  csr_free(A);
+  return 0;
+}
+
+int fx_csr_info(fx_csr A, uint32_t* nrow, uint32_t* mcol, uint32_t* nnz) {
+  clear_error();
+  FX_CHECK(A, FX_ERROR_INPUT_PAR, "null matrix");
+  if (nrow) *nrow = A->nrow;
+  if (mcol) *mcol = A->mcol;
+  if (nnz) *nnz = A->nnz;
+  return 0;
+}
+
+int fx_csr_download(fx_csr A, uint32_t* rowPtr, uint32_t* colInd, double* values,
+                    uint32_t* diagPtr) {
+  clear_error();
+  FX_CHECK(A, FX_ERROR_INPUT_PAR, "null matrix");
+  if (A->nrow == 0) return 0;
+  hipStream_t st = A->ctx->stream;
+  if (rowPtr)
+    FX_HIP(hipMemcpyAsync(rowPtr, A->rp, sizeof(uint32_t) * (A->nrow + 1), hipMemcpyDeviceToHost, st));
+  if (colInd)
+    FX_HIP(hipMemcpyAsync(colInd, A->ci, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost, st));
+  if (values && A->has_values)
+    FX_HIP(hipMemcpyAsync(values, A->va, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, st));
+  if (diagPtr) {
+    const uint32_t nd = A->nrow < A->mcol ? A->nrow : A->mcol;
+    FX_HIP(hipMemcpyAsync(diagPtr, A->dp, sizeof(uint32_t) * nd, hipMemcpyDeviceToHost, st));
+  }
+  FX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int fx_csr_make_f32(fx_csr A) {
+  clear_error();
+  FX_CHECK(A, FX_ERROR_INPUT_PAR, "null matrix");
+  if (A->nnz == 0 || !A->has_values) return 0;
+  if (!A->va32) FX_HIP(hipMalloc(&A->va32, sizeof(float) * A->nnz));
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(stream_blocks(A->nnz)), dim3(kThreads), 0, A->ctx->stream,
+                     A->va, A->va32, A->nnz);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fx_csr_set_values_dev(fx_csr A, const double* dev_values) {
+  clear_error();
+  FX_CHECK(A && dev_values, FX_ERROR_INPUT_PAR, "null argument");
+  if (A->nnz == 0) return 0;
+  FX_HIP(hipMemcpyAsync(A->va, dev_values, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice,
+                        A->ctx->stream));
+  A->has_values = true;
+  if (A->va32) return fx_csr_make_f32(A);
+  return 0;
+}
+
+double* fx_csr_values_dev(fx_csr A) { return A ? A->va : nullptr; }
+
+int fx_spmv(fx_ctx ctx, fx_csr A, fx_vec v, fx_vec w) {
+  clear_error();
+  FX_CHECK(ctx && A && v && w, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(v->n == A->mcol && w->n == A->nrow, FX_ERROR_NONMATCH_SIZE,
+           "SpMV sizes: A %ux%u, v %u, w %u", A->nrow, A->mcol, v->n, w->n);
+  FX_CHECK(v->dtype == w->dtype, FX_ERROR_INPUT_PAR, "SpMV vector dtypes differ");
+  FX_CHECK(v->d != w->d || A->nrow == 0, FX_ERROR_INPUT_PAR, "SpMV in place is not supported");
+  return csr_spmv_dispatch(ctx, A, v->d, w->d, v->dtype);
+}
+
+int fx_residual(fx_ctx ctx, fx_csr A, fx_vec b, fx_vec x, fx_vec r) {
+  clear_error();
+  FX_CHECK(ctx && A && b && x && r, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(x->n == A->mcol && b->n == A->nrow && r->n == A->nrow, FX_ERROR_NONMATCH_SIZE,
+           "Residual sizes mismatch");
+  FX_CHECK(b->dtype == x->dtype && x->dtype == r->dtype, FX_ERROR_INPUT_PAR, "dtypes differ");
+  FX_CHECK(r->d != x->d, FX_ERROR_INPUT_PAR, "Residual with r aliasing x is not supported");
+  if (A->nrow == 0) return 0;
+  hipStream_t st = ctx->stream;
+  const unsigned g = stream_blocks(x->n);
+  double* xinf = ctx->dscal + 1;
+  if (x->dtype == FX_F64) {
+    hipLaunchKernelGGL(k_norminf<double>, dim3(g), dim3(kThreads), 0, st, (const double*)x->d, x->n,
+                       ctx->ws.partials, ctx->ws.counter, xinf);
+    return launch_spmv<double, double, ROW_B_MINUS_AX, 0, 0u>(
+        A, st, XLoad<double>{(const double*)x->d}, BLoad<double>{(const double*)b->d},
+        EpiResidual<double>{(double*)r->d, (const double*)b->d, xinf}, nullptr, nullptr);
+  }
+  FX_CHECK(A->va32 || !A->has_values, FX_ERROR_INPUT_PAR, "f32 residual needs fx_csr_make_f32");
+  hipLaunchKernelGGL(k_norminf<float>, dim3(g), dim3(kThreads), 0, st, (const float*)x->d, x->n,
+                     ctx->ws.partials, ctx->ws.counter, xinf);
+  return launch_spmv<float, float, ROW_B_MINUS_AX, 0, 0u>(
+      A, st, XLoad<float>{(const float*)x->d}, BLoad<float>{(const float*)b->d},
+      EpiResidual<float>{(float*)r->d, (const float*)b->d, xinf}, nullptr, nullptr);
+}
+
+int fx_csr_get_diag(fx_ctx ctx, fx_csr A, fx_vec v) {
+  clear_error();
+  FX_CHECK(ctx && A && v, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(v->n == A->nrow, FX_ERROR_NONMATCH_SIZE, "GetDiag: vector size %u != nrow %u", v->n,
+           A->nrow);
+  FX_CHECK(A->nrow <= A->mcol, FX_ERROR_MAT_SIZE, "GetDiag needs nrow <= mcol");
+  if (A->nrow == 0) return 0;
+  const unsigned g = (A->nrow + kThreads - 1) / kThreads;
+  const double* va = A->has_values ? A->va : nullptr;
+  if (v->dtype == FX_F64)
+    hipLaunchKernelGGL(k_get_diag<double>, dim3(g), dim3(kThreads), 0, ctx->stream, va, A->dp,
+                       (double*)v->d, A->nrow);
+  else
+    hipLaunchKernelGGL(k_get_diag<float>, dim3(g), dim3(kThreads), 0, ctx->stream, va, A->dp,
+                       (float*)v->d, A->nrow);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At) {
+  clear_error();
+  FX_CHECK(ctx && A && At, FX_ERROR_INPUT_PAR, "null argument");
+  fx_csr T = new fx_csr_s();
+  T->ctx = ctx;
+  if (A->nnz == 0) {
+    *At = T;
+    return 0;
+  }
+  T->nrow = A->mcol;
+  T->mcol = A->nrow;
+  T->nnz = A->nnz;
+  T->has_values = A->has_values;
+  int rc = csr_alloc(T);
+  if (rc) {
+    csr_free(T);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  const uint32_t m = A->mcol, nnz = A->nnz;
+  uint32_t *cnt = nullptr, *perm = nullptr, *sorted = nullptr, *rid = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(cnt);
+    (void)hipFree(perm);
+    (void)hipFree(sorted);
+    (void)hipFree(rid);
+  };
+  if (hipMalloc(&cnt, sizeof(uint32_t) * (m + 1)) || hipMalloc(&perm, sizeof(uint32_t) * nnz) ||
+      hipMalloc(&sorted, sizeof(uint32_t) * nnz) || hipMalloc(&rid, sizeof(uint32_t) * nnz)) {
+    cleanup();
+    csr_free(T);
+    set_error("transpose: allocation failed");
+    return FX_ERROR_ALLOC_MEM;
+  }
+  const unsigned gz = stream_blocks(nnz);
+  (void)hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (m + 1), st);
+  hipLaunchKernelGGL(k_col_count, dim3(gz), dim3(kThreads), 0, st, A->ci, nnz, cnt);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cnt, m, T->rp);
+  (void)hipMemcpyAsync(cnt, T->rp, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(k_scatter, dim3(gz), dim3(kThreads), 0, st, A->ci, nnz, cnt, perm);
+  hipLaunchKernelGGL(k_rank_rows, dim3((m + kWaves - 1) / kWaves), dim3(kThreads), 0, st, T->rp, m,
+                     perm, sorted);
+  hipLaunchKernelGGL(k_row_ids, dim3((A->nrow + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                     A->rp, A->nrow, rid);
+  hipLaunchKernelGGL(k_gather_t, dim3(gz), dim3(kThreads), 0, st, sorted, nnz, rid,
+                     A->has_values ? A->va : nullptr, T->ci, T->has_values ? T->va : nullptr);
+  hipLaunchKernelGGL(k_form_diagptr, dim3((m + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                     T->rp, T->ci, T->dp, m);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    cleanup();
+    csr_free(T);
+    set_error("transpose kernels: %s", hipGetErrorString(e));
+    return FX_ERROR_DEVICE;
+  }
+  std::vector<uint32_t> hrp(m + 1);
+  if (hipMemcpyAsync(hrp.data(), T->rp, sizeof(uint32_t) * (m + 1), hipMemcpyDeviceToHost, st) ||
+      hipStreamSynchronize(st)) {
+    cleanup();
+    csr_free(T);
+    set_error("transpose: download failed");
+    return FX_ERROR_DEVICE;
+  }
+  cleanup();
+  rc = csr_set_blocks(T, hrp.data());
+  if (rc) {
+    csr_free(T);
+    return rc;
+  }
+  if (A->va32) {
+    rc = fx_csr_make_f32(T);
+    if (rc) {
+      csr_free(T);
+      return rc;
+    }
+  }
+  *At = T;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,964 @@
+// fx_fast.hip -- fused, graph-replayed solvers.
+//
+// Jacobi-preconditioned (one sweep, any weight) or unpreconditioned CG,
+// the weighted-Jacobi solver, and the mixed-precision refinement built on
+// the f32 CG.  All scalars stay in a device state block; the reference's
+// per-iteration decisions (src/CG.cxx:82-236, src/Iter.cxx:69-99) run in the
+// finisher of each reduction kernel.  The rare branches (CG's breakdown
+// guard, stagnation checks I-II, the false-convergence check III, the
+// safe-guard copy) stop the device loop and are run on the host with the
+// per-operation backend, after which the device loop resumes.
+//
+// CG iteration = 3 kernels, replayed as a hipGraph of kChunk iterations:
+//   K1  q = A p, (q, p)                 -> alpha            (SpMV + 1 reduction)
+//   K2  x += alpha p, r -= alpha q, z = D r;  (r,r) (z,r) max|x| (x,x) (p,p)
+//                                       -> checks, beta     (stream + 5 reductions)
+//   K3  p = beta p + z                                       (stream)
+// Per iteration HBM traffic: SpMV + 13 vector passes (K1 2, K2 7, K3 4 in f64
+// units: p/q, x p r q D / x r, r D p / p).
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "fx_solver.hpp"
+#include "fx_spmv.hpp"
+
+namespace fx {
+
+bool fast_disabled() {
+  const char* e = std::getenv("FASPXX_FX_NO_FAST");
+  return e && e[0] == '1';
+}
+
+namespace {
+
+constexpr int kChunk = 8;  // iterations per graph replay
+
+enum { ST_RUN = 0, ST_DONE = 1, ST_HOST = 2 };
+enum { R_BREAKDOWN = 1, R_CHECKS = 2 };
+
+struct CgState {
+  double tmpa, pq, alpha, beta, resAbs, resAbsOld, resRel, denAbs, ratio;
+  double rr, rz, xinf, xx, pp;
+  uint32_t numIter, stagStep, moreStep, histCount;
+  int status, reason, err, pad;
+};
+
+struct CgPrm {
+  uint32_t maxIter, minIter, savIter, pad;
+  double relTol, absTol, solStagTol;
+};
+
+// ---- CG kernels -----------------------------------------------------------
+
+template <class VT>
+struct EpiBMinusW {  // r = b - (A x), the Apply + XPAY(-1, b) of src/CG.cxx:70-71
+  VT* __restrict__ r;
+  const VT* __restrict__ b;
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&)[1]) const {
+    const VT w = (VT)v;
+    r[i] = b[i] - w;
+  }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+template <class VT>
+struct EpiCgQ {  // q = A p and (q, p) -> alpha   (src/CG.cxx:99-119)
+  VT* __restrict__ q;
+  const VT* __restrict__ p;
+  CgState* st;
+  __device__ bool skip() const { return st->status != ST_RUN; }
+  __device__ __forceinline__ void row(uint32_t r, double v, double (&acc)[1]) const {
+    const VT qv = (VT)v;
+    q[r] = qv;
+    acc[0] = fma((double)qv, (double)p[r], acc[0]);
+  }
+  __device__ void fin(double (&s)[1]) const {
+    if (threadIdx.x != 0) return;
+    CgState* S = st;
+    S->numIter += 1;
+    S->pq = s[0];
+    if (fabs(s[0]) > kCloseZero * kCloseZero) {
+      S->alpha = S->tmpa / s[0];
+    } else {
+      S->status = ST_HOST;
+      S->reason = R_BREAKDOWN;
+    }
+  }
+};
+
+// Loop-top bookkeeping (src/CG.cxx:84-93) for iteration `it`.
+__device__ void cg_loop_top(CgState* S, const CgPrm& P, double rr, double* hist) {
+  const uint32_t it = S->numIter;
+  if (it == P.minIter) {
+    const double resAbs = sqrt(rr);
+    S->resAbs = resAbs;
+    S->resAbsOld = resAbs;
+    S->denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+    S->resRel = resAbs / S->denAbs;
+    if (S->resRel < P.relTol || resAbs < P.absTol) {
+      S->status = ST_DONE;
+      return;
+    }
+  }
+  if (it >= P.minIter) {
+    hist[it] = S->resAbs;
+    S->histCount = it + 1;
+  }
+}
+
+// End-of-iteration logic of src/CG.cxx:130-234 for the common case; any rare
+// branch hands the whole step to the host (which recomputes it from S).
+__device__ void cg_after_update(CgState* S, const CgPrm& P, const double (&a)[5], double* hist) {
+  S->rr = a[0];
+  S->rz = a[1];
+  S->xinf = a[2];
+  S->xx = a[3];
+  S->pp = a[4];
+  const uint32_t it = S->numIter;
+  if (it >= P.minIter) {
+    const double resAbs = sqrt(a[0]);
+    const double resRel = resAbs / S->denAbs;
+    const double ratio = resAbs / S->resAbsOld;
+    bool host = false;
+    if (it >= P.savIter && resAbs < S->resAbsOld) host = true;
+    if (ratio > kChkRatio) {
+      if (a[2] < kCloseZero) host = true;
+      const double xRelDiff = fabs(S->alpha) * sqrt(a[4]) / sqrt(a[3]);
+      if (S->stagStep <= kMaxStag && xRelDiff < P.solStagTol) host = true;
+    }
+    if (resRel < P.relTol) host = true;
+    if (host) {
+      S->status = ST_HOST;
+      S->reason = R_CHECKS;
+      return;
+    }
+    S->resAbs = resAbs;
+    S->resRel = resRel;
+    S->ratio = ratio;
+  }
+  if (it < P.maxIter) {
+    S->resAbsOld = S->resAbs;
+    S->beta = a[1] / S->tmpa;
+    S->tmpa = a[1];
+  } else {
+    S->status = ST_DONE;
+    return;
+  }
+  cg_loop_top(S, P, a[0], hist);
+}
+
+template <class VT, bool JAC>
+__global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* __restrict__ r,
+                                                        const VT* __restrict__ p,
+                                                        const VT* __restrict__ q,
+                                                        const VT* __restrict__ dinv, uint32_t n,
+                                                        CgState* st, CgPrm P, double* hist,
+                                                        double* partials, unsigned* counter) {
+  if (st->status != ST_RUN) return;
+  const VT a = (VT)st->alpha;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const VT pi = p[i], qi = q[i];
+    const VT xi = fma(a, pi, x[i]);   // x.AXPY(alpha, p)
+    const VT ri = fma(-a, qi, r[i]);  // r.AXPY(-alpha, q)
+    x[i] = xi;
+    r[i] = ri;
+    const VT zi = JAC ? ri * dinv[i] : ri;  // Jacobi sweep from z = 0: z = r .* dinv
+    acc[0] = fma((double)ri, (double)ri, acc[0]);
+    acc[1] = fma((double)zi, (double)ri, acc[1]);
+    acc[2] = fmax(acc[2], fabs((double)xi));
+    acc[3] = fma((double)xi, (double)xi, acc[3]);
+    acc[4] = fma((double)pi, (double)pi, acc[4]);
+  }
+  if (grid_reduce<5, 4u>(acc, partials, counter) && threadIdx.x == 0)
+    cg_after_update(st, P, acc, hist);
+}
+
+template <class VT, bool JAC>
+__global__ __launch_bounds__(kThreads) void k_cg_pupdate(VT* __restrict__ p,
+                                                         const VT* __restrict__ r,
+                                                         const VT* __restrict__ dinv, uint32_t n,
+                                                         const CgState* st) {
+  if (st->status != ST_RUN) return;
+  const VT beta = (VT)st->beta;
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const VT zi = JAC ? r[i] * dinv[i] : r[i];
+    p[i] = fma(beta, p[i], zi);  // p.XPAY(beta, z)
+  }
+}
+
+// z = P(r), p = z, tmpa = (z, r), and the first loop-top check.
+template <class VT, bool JAC>
+__global__ __launch_bounds__(kThreads) void k_cg_init(VT* __restrict__ p, const VT* __restrict__ r,
+                                                      const VT* __restrict__ dinv, uint32_t n,
+                                                      CgState* st, CgPrm P, double* hist,
+                                                      double* partials, unsigned* counter) {
+  double acc[2] = {0.0, 0.0};
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const VT ri = r[i];
+    const VT zi = JAC ? ri * dinv[i] : ri;
+    p[i] = zi;
+    acc[0] = fma((double)ri, (double)ri, acc[0]);
+    acc[1] = fma((double)zi, (double)ri, acc[1]);
+  }
+  if (grid_reduce<2>(acc, partials, counter) && threadIdx.x == 0) {
+    CgState* S = st;
+    S->tmpa = acc[1];
+    S->rr = acc[0];
+    if (P.maxIter == 0) {
+      S->status = ST_DONE;
+      return;
+    }
+    cg_loop_top(S, P, acc[0], hist);
+  }
+}
+
+// ---- Jacobi solver kernels (src/Iter.cxx:69-99) ---------------------------
+
+struct JacState {
+  double resAbs, resRel, denAbs, xinf;
+  uint32_t numIter, histCount, answer, pad;  // answer: 0 -> buffer A, 1 -> buffer B
+  int status, pad2;
+};
+
+// One sweep reading x (buffer `src`) and writing x+ (buffer `dst`):
+// work = b - A x (or b when ||x||_inf < 1e-20), ||work||, x+ = x + work .* dinv.
+struct EpiJacobi {
+  double* __restrict__ xn;
+  const double* __restrict__ x;
+  const double* __restrict__ b;
+  const double* __restrict__ dinv;
+  JacState* st;
+  CgPrm P;
+  uint32_t self;  // index of `x` (0 A / 1 B)
+  double* hist;
+  __device__ bool skip() const { return st->status != ST_RUN; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
+    const double w = (st->xinf < kCloseZero) ? b[i] : v;
+    acc[0] = fma(w, w, acc[0]);
+    const double wd = w * dinv[i];  // work.PointwiseMult(diagInv)
+    const double xv = x[i] + wd;    // x += work
+    xn[i] = xv;
+    acc[1] = fmax(acc[1], fabs(xv));
+  }
+  __device__ void fin(double (&s)[2]) const {
+    if (threadIdx.x != 0) return;
+    JacState* S = st;
+    const uint32_t it = S->numIter;
+    if (it >= P.minIter) {
+      const double resAbs = sqrt(s[0]);
+      if (it == P.minIter) S->denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+      S->resAbs = resAbs;
+      S->resRel = resAbs / S->denAbs;
+      if (S->resRel < P.relTol || resAbs < P.absTol) {
+        S->status = ST_DONE;
+        S->answer = self;  // x was not updated in this sweep
+        return;
+      }
+      hist[it] = resAbs;
+      S->histCount = it + 1;
+    }
+    S->numIter = it + 1;
+    S->xinf = s[1];
+    S->answer = 1u - self;
+    if (S->numIter >= P.maxIter) S->status = ST_DONE;
+  }
+};
+
+// Final residual norms (src/Iter.cxx:102-109): ||b - A x||_2 and _inf.
+struct EpiResNorms {
+  const double* __restrict__ b;
+  const double* xinf;
+  double* out;  // [norm2, normInf]
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
+    const double w = (*xinf < kCloseZero) ? b[i] : v;
+    acc[0] = fma(w, w, acc[0]);
+    acc[1] = fmax(acc[1], fabs(w));
+  }
+  __device__ void fin(double (&s)[2]) const {
+    if (threadIdx.x == 0) {
+      out[0] = sqrt(s[0]);
+      out[1] = s[1];
+    }
+  }
+};
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_norminf_state(const T* __restrict__ x, uint32_t n,
+                                                            double* out, double* partials,
+                                                            unsigned* counter) {
+  double acc[1] = {0.0};
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    acc[0] = fmax(acc[0], fabs((double)x[i]));
+  if (grid_reduce<1, 1u>(acc, partials, counter) && threadIdx.x == 0) *out = acc[0];
+}
+
+// ---- mixed precision helpers -----------------------------------------------
+
+// r = b - A x in f64 (Apply + XPAY order), stored as f32; ||r||^2 -> out.
+struct EpiResidF32 {
+  float* __restrict__ r32;
+  const double* __restrict__ b;
+  double* out;
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[1]) const {
+    const double w = b[i] - v;
+    r32[i] = (float)w;
+    acc[0] = fma(w, w, acc[0]);
+  }
+  __device__ void fin(double (&s)[1]) const {
+    if (threadIdx.x == 0) *out = sqrt(s[0]);
+  }
+};
+
+__global__ void k_add_f32_to_f64(double* __restrict__ x, const float* __restrict__ d, uint32_t n) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    x[i] = x[i] + (double)d[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Workspace
+// ---------------------------------------------------------------------------
+struct FastWs {
+  fx_ctx ctx = nullptr;
+  int dt = FX_F64;
+  uint32_t n = 0;
+  void* x = nullptr;  // internal solution / Jacobi ping buffer
+  void* x2 = nullptr; // Jacobi pong buffer
+  void* r = nullptr;
+  void* p = nullptr;
+  void* q = nullptr;
+  void* safe = nullptr;
+  double* partials = nullptr;
+  unsigned* counters = nullptr;
+  void* state = nullptr;  // CgState / JacState
+  void* hstate = nullptr; // pinned, 2 slots
+  double* hist = nullptr;
+  uint32_t hist_cap = 0;
+  double* dscal = nullptr;
+  hipGraphExec_t chunk = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  uint32_t n_cg_updates = 0;
+
+  ~FastWs() {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    void* bufs[] = {x, x2, r, p, q, safe, partials, counters, state, hist, dscal};
+    for (void* b : bufs) (void)hipFree(b);
+    if (hstate) (void)hipHostFree(hstate);
+    if (chunk) (void)hipGraphExecDestroy(chunk);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_t state_bytes,
+             uint32_t spmv_blocks) {
+  W.ctx = ctx;
+  W.dt = dt;
+  W.n = n;
+  const size_t vb = (size_t)(n ? n : 1) * (dt == FX_F64 ? 8 : 4);
+  FX_HIP(hipMalloc(&W.x, vb));
+  FX_HIP(hipMalloc(&W.x2, vb));
+  FX_HIP(hipMalloc(&W.r, vb));
+  FX_HIP(hipMalloc(&W.p, vb));
+  FX_HIP(hipMalloc(&W.q, vb));
+  FX_HIP(hipMalloc(&W.safe, vb));
+  FX_HIP(hipMemsetAsync(W.safe, 0, vb, ctx->stream));
+  const size_t np = 8 * (size_t)((spmv_blocks > (uint32_t)kMaxRedBlocks) ? spmv_blocks : kMaxRedBlocks);
+  FX_HIP(hipMalloc(&W.partials, sizeof(double) * np));
+  FX_HIP(hipMalloc(&W.counters, sizeof(unsigned) * 16));
+  FX_HIP(hipMemsetAsync(W.counters, 0, sizeof(unsigned) * 16, ctx->stream));
+  FX_HIP(hipMalloc(&W.state, state_bytes));
+  FX_HIP(hipHostMalloc(&W.hstate, 2 * state_bytes, hipHostMallocDefault));
+  W.hist_cap = hist_cap + 2;
+  FX_HIP(hipMalloc(&W.hist, sizeof(double) * W.hist_cap));
+  FX_HIP(hipMalloc(&W.dscal, sizeof(double) * 8));
+  FX_HIP(hipEventCreateWithFlags(&W.ev[0], hipEventDisableTiming));
+  FX_HIP(hipEventCreateWithFlags(&W.ev[1], hipEventDisableTiming));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CG
+// ---------------------------------------------------------------------------
+template <class VT>
+struct CgRun {
+  fx_solver S;
+  FastWs& W;
+  fx_csr A;
+  const VT* dinv;  // null: identity preconditioner
+  CgPrm P;
+  hipStream_t st;
+
+  CgState* dstate() const { return (CgState*)W.state; }
+
+  int launch_iteration() {
+    VT* p = (VT*)W.p;
+    VT* q = (VT*)W.q;
+    VT* x = (VT*)W.x;
+    VT* r = (VT*)W.r;
+    const uint32_t n = W.n;
+    FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
+                                               EpiCgQ<VT>{q, p, dstate()}, W.partials,
+                                               W.counters + 0)));
+    const unsigned g = stream_blocks(n);
+    if (dinv) {
+      hipLaunchKernelGGL((k_cg_update<VT, true>), dim3(g), dim3(kThreads), 0, st, x, r, p, q, dinv,
+                         n, dstate(), P, W.hist, W.partials, W.counters + 1);
+      hipLaunchKernelGGL((k_cg_pupdate<VT, true>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
+                         dstate());
+    } else {
+      hipLaunchKernelGGL((k_cg_update<VT, false>), dim3(g), dim3(kThreads), 0, st, x, r, p, q,
+                         dinv, n, dstate(), P, W.hist, W.partials, W.counters + 1);
+      hipLaunchKernelGGL((k_cg_pupdate<VT, false>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
+                         dstate());
+    }
+    FX_HIP(hipGetLastError());
+    return 0;
+  }
+
+  int build_graph() {
+    if (W.chunk) return 0;
+    hipGraph_t g = nullptr;
+    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int c = 0; c < kChunk && !rc; ++c) rc = launch_iteration();
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) return rc;
+    FX_HIP(e);
+    e = hipGraphInstantiate(&W.chunk, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FX_HIP(e);
+    return 0;
+  }
+
+  int launch_chunk(int slot) {
+    FX_HIP(hipGraphLaunch(W.chunk, st));
+    FX_HIP(hipMemcpyAsync((CgState*)W.hstate + slot, W.state, sizeof(CgState),
+                          hipMemcpyDeviceToHost, st));
+    FX_HIP(hipEventRecord(W.ev[slot], st));
+    return 0;
+  }
+
+  int read_state(CgState& s) {
+    FX_HIP(hipStreamSynchronize(st));
+    FX_HIP(hipMemcpy(&s, W.state, sizeof(CgState), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  int write_state(const CgState& s) {
+    FX_HIP(hipMemcpyAsync(W.state, &s, sizeof(CgState), hipMemcpyHostToDevice, st));
+    FX_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int write_hist(uint32_t it, double v, CgState& s) {
+    if (it >= W.hist_cap) return 0;
+    FX_HIP(hipMemcpy(W.hist + it, &v, sizeof(double), hipMemcpyHostToDevice));
+    s.histCount = it + 1;
+    return 0;
+  }
+
+  // The rare branches, stated as in src/CG.cxx:104-234, from the state the
+  // device left.  Returns 1 if the solve ends, 0 to resume the device loop.
+  int host_step(CgState& s, fx_vec vb, fx_vec vx, fx_vec vr, fx_vec vp, fx_vec vq, fx_vec vsafe,
+                Ops& o, const fx_params& PP) {
+    const double solStagTol = P.solStagTol;
+    if (s.reason == R_BREAKDOWN) {
+      o.axpy(s.alpha, vp, vx);
+      o.axpy(-s.alpha, vq, vr);
+      s.resAbs = o.norm2(vr);
+      s.resRel = s.resAbs / s.denAbs;
+      if (s.resRel > PP.relTol && s.resAbs > PP.absTol) s.err = FX_ERROR_DIVIDE_ZERO;
+      return 1;
+    }
+    const uint32_t it = s.numIter;
+    double resAbs = s.resAbs, resRel = s.resRel;
+    if (it >= PP.minIter) {
+      resAbs = std::sqrt(s.rr);
+      resRel = resAbs / s.denAbs;
+      const double ratio = resAbs / s.resAbsOld;
+      s.ratio = ratio;
+      if (it >= PP.savIter && resAbs < s.resAbsOld) o.copy(vx, vsafe);
+      if (ratio > kChkRatio) {
+        if (s.xinf < kCloseZero) {
+          s.err = FX_ERROR_SOLVER_SOLSTAG;
+          s.resAbs = resAbs;
+          s.resRel = resRel;
+          return 1;
+        }
+        const double xRelDiff = std::fabs(s.alpha) * std::sqrt(s.pp) / std::sqrt(s.xx);
+        if (s.stagStep <= kMaxStag && xRelDiff < solStagTol) {
+          o.true_residual(vb, vx, vr);
+          resAbs = o.norm2(vr);
+          resRel = resAbs / s.denAbs;
+          if (resRel < PP.relTol || resAbs < PP.absTol) {
+            s.resAbs = resAbs;
+            s.resRel = resRel;
+            return 1;
+          }
+          if (s.stagStep >= kMaxStag) {
+            s.err = FX_ERROR_SOLVER_STAG;
+            s.resAbs = resAbs;
+            s.resRel = resRel;
+            return 1;
+          }
+          o.set(vp, 0.0);
+          ++s.stagStep;
+        }
+      }
+      if (resRel < PP.relTol) {
+        o.true_residual(vb, vx, vr);
+        resAbs = o.norm2(vr);
+        resRel = resAbs / s.denAbs;
+        if (resRel < PP.relTol || resAbs < PP.absTol) {
+          s.resAbs = resAbs;
+          s.resRel = resRel;
+          return 1;
+        }
+        if (s.moreStep >= PP.restart) {
+          s.err = FX_ERROR_SOLVER_TOLSMALL;
+          s.resAbs = resAbs;
+          s.resRel = resRel;
+          return 1;
+        }
+        o.set(vp, 0.0);
+        ++s.moreStep;
+      }
+      s.resAbs = resAbs;
+      s.resRel = resRel;
+    }
+    if (it < PP.maxIter) {
+      s.resAbsOld = s.resAbs;
+      // z = P(r) into q (q is free until the next SpMV), tmpb = (z, r)
+      o.set(vq, 0.0);
+      o.precond(vr, vq);
+      const double tmpb = o.dot(vq, vr);
+      s.beta = tmpb / s.tmpa;
+      s.tmpa = tmpb;
+      o.xpay(s.beta, vq, vp);
+    } else {
+      return 1;
+    }
+    // loop top
+    if (it == PP.minIter) {
+      const double ra = o.norm2(vr);
+      s.resAbs = ra;
+      s.resAbsOld = ra;
+      s.denAbs = (kCloseZero > ra) ? kCloseZero : ra;
+      s.resRel = ra / s.denAbs;
+      if (s.resRel < PP.relTol || ra < PP.absTol) return 1;
+    }
+    if (it >= PP.minIter) FX_TRY(write_hist(it, s.resAbs, s));
+    s.status = ST_RUN;
+    s.reason = 0;
+    return 0;
+  }
+
+  // x_user (in/out), b_user: device pointers of dtype VT.
+  int run(const VT* b, VT* x_user, fx_result* res, const fx_params& PP, int pc_dtype) {
+    const uint32_t n = W.n;
+    FX_HIP(hipMemcpyAsync(W.x, x_user, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
+    CgState s0{};
+    s0.resAbs = 1.0;
+    s0.resRel = 1.0;
+    s0.denAbs = 1.0;
+    s0.ratio = 0.0;
+    s0.resAbsOld = 1.0;
+    s0.alpha = 0.0;
+    s0.status = ST_RUN;
+    FX_TRY(write_state(s0));
+    // r = b - A x;  z = P r;  p = z;  tmpa = (z, r);  loop top
+    FX_TRY((launch_spmv<VT, VT, ROW_AX, 0, 0u>(A, st, XLoad<VT>{(const VT*)W.x}, BNone{},
+                                               EpiBMinusW<VT>{(VT*)W.r, b}, nullptr, nullptr)));
+    const unsigned g = stream_blocks(n);
+    if (dinv)
+      hipLaunchKernelGGL((k_cg_init<VT, true>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
+                         (const VT*)W.r, dinv, n, dstate(), P, W.hist, W.partials, W.counters + 2);
+    else
+      hipLaunchKernelGGL((k_cg_init<VT, false>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
+                         (const VT*)W.r, dinv, n, dstate(), P, W.hist, W.partials, W.counters + 2);
+    FX_HIP(hipGetLastError());
+    FX_TRY(build_graph());
+
+    // Device-vector views for the host branches
+    Ops o(W.ctx, A, S->pc, pc_dtype);
+    fx_vec vb = nullptr, vx = nullptr, vr = nullptr, vp = nullptr, vq = nullptr, vs = nullptr;
+    const int dt = pc_dtype;
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, (void*)b, &vb));
+    o.owned.push_back(vb);
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.x, &vx));
+    o.owned.push_back(vx);
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.r, &vr));
+    o.owned.push_back(vr);
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.p, &vp));
+    o.owned.push_back(vp);
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.q, &vq));
+    o.owned.push_back(vq);
+    Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.safe, &vs));
+    o.owned.push_back(vs);
+
+    CgState s{};
+    // Every replay advances the device loop by kChunk iterations unless it has
+    // stopped, so the loop must stop within maxIter / kChunk replays.
+    const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
+    uint64_t chunks = 0;
+    for (;;) {
+      FX_TRY(launch_chunk(0));
+      FX_TRY(launch_chunk(1));
+      chunks += 2;
+      int cur = 0;
+      for (;;) {
+        FX_HIP(hipEventSynchronize(W.ev[cur]));
+        const CgState* hs = (const CgState*)W.hstate + cur;
+        if (hs->status != ST_RUN) break;
+        FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "CG device loop did not stop");
+        FX_TRY(launch_chunk(cur));
+        ++chunks;
+        cur ^= 1;
+      }
+      FX_TRY(read_state(s));
+      if (s.status == ST_DONE) break;
+      // ST_HOST
+      const int fin = host_step(s, vb, vx, vr, vp, vq, vs, o, PP);
+      if (fin) break;
+      FX_TRY(write_state(s));
+    }
+    res->numIter = s.numIter;
+    if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
+      res->norm2 = s.resAbs;
+      res->normInf = o.norminf(vr);
+    }
+    if (s.numIter > PP.savIter) o.copy(vs, vx);
+    FX_HIP(hipMemcpyAsync(x_user, W.x, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
+    // history
+    const uint32_t hc = s.histCount < W.hist_cap ? s.histCount : W.hist_cap;
+    S->hist_host.assign(hc, 0.0);
+    if (hc) FX_HIP(hipMemcpy(S->hist_host.data(), W.hist, sizeof(double) * hc, hipMemcpyDeviceToHost));
+    S->hist_count = hc;
+    FX_HIP(hipStreamSynchronize(st));
+    return s.err;
+  }
+};
+
+CgPrm make_prm(const fx_params& p) {
+  CgPrm c{};
+  c.maxIter = p.maxIter;
+  c.minIter = p.minIter;
+  c.savIter = p.savIter;
+  c.relTol = p.relTol;
+  c.absTol = p.absTol;
+  c.solStagTol = 1e-4 * p.relTol;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi solver
+// ---------------------------------------------------------------------------
+int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_result* res) {
+  const fx_params& PP = S->prm;
+  const CgPrm P = make_prm(PP);
+  hipStream_t st = W.ctx->stream;
+  fx_csr A = S->A;
+  const uint32_t n = W.n;
+  JacState* ds = (JacState*)W.state;
+  double* xa = (double*)W.x;
+  double* xb = (double*)W.x2;
+  FX_HIP(hipMemcpyAsync(xa, x_user, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  JacState s0{};
+  s0.resAbs = 1.0;
+  s0.resRel = 1.0;
+  s0.denAbs = 1.0;
+  s0.status = P.maxIter == 0 ? ST_DONE : ST_RUN;
+  FX_HIP(hipMemcpyAsync(ds, &s0, sizeof(JacState), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_norminf_state<double>, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                     (const double*)xa, n, &ds->xinf, W.partials, W.counters + 3);
+  FX_HIP(hipGetLastError());
+  const double* dinv = S->pc->dinv;
+  if (!W.chunk) {
+    hipGraph_t g = nullptr;
+    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int c = 0; c < kChunk && !rc; ++c) {
+      const bool even = (c % 2) == 0;
+      const double* src = even ? xa : xb;
+      double* dst = even ? xb : xa;
+      rc = launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
+          A, st, XLoad<double>{src}, BLoad<double>{b},
+          EpiJacobi{dst, src, b, dinv, ds, P, even ? 0u : 1u, W.hist}, W.partials, W.counters + 4);
+    }
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) return rc;
+    FX_HIP(e);
+    e = hipGraphInstantiate(&W.chunk, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FX_HIP(e);
+  }
+  // kChunk is even: every replay starts reading buffer A
+  JacState s{};
+  int cur = 0;
+  for (int i = 0; i < 2; ++i) {
+    FX_HIP(hipGraphLaunch(W.chunk, st));
+    FX_HIP(hipMemcpyAsync((JacState*)W.hstate + i, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipEventRecord(W.ev[i], st));
+  }
+  const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
+  uint64_t chunks = 2;
+  for (;;) {
+    FX_HIP(hipEventSynchronize(W.ev[cur]));
+    if (((const JacState*)W.hstate + cur)->status != ST_RUN) break;
+    FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "Jacobi device loop did not stop");
+    ++chunks;
+    FX_HIP(hipGraphLaunch(W.chunk, st));
+    FX_HIP(hipMemcpyAsync((JacState*)W.hstate + cur, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipEventRecord(W.ev[cur], st));
+    cur ^= 1;
+  }
+  FX_HIP(hipStreamSynchronize(st));
+  FX_HIP(hipMemcpy(&s, ds, sizeof(JacState), hipMemcpyDeviceToHost));
+  double* ans = s.answer ? xb : xa;
+  res->numIter = s.numIter;
+  if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
+    hipLaunchKernelGGL(k_norminf_state<double>, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                       (const double*)ans, n, W.dscal, W.partials, W.counters + 5);
+    FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
+        A, st, XLoad<double>{ans}, BLoad<double>{b}, EpiResNorms{b, W.dscal, W.dscal + 1},
+        W.partials, W.counters + 6)));
+    double h[3];
+    FX_HIP(hipMemcpyAsync(h, W.dscal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    res->norm2 = h[1];
+    res->normInf = h[2];
+  }
+  FX_HIP(hipMemcpyAsync(x_user, ans, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  const uint32_t hc = s.histCount < W.hist_cap ? s.histCount : W.hist_cap;
+  S->hist_host.assign(hc, 0.0);
+  if (hc) FX_HIP(hipMemcpy(S->hist_host.data(), W.hist, sizeof(double) * hc, hipMemcpyDeviceToHost));
+  S->hist_count = hc;
+  FX_HIP(hipStreamSynchronize(st));
+  return FX_SUCCESS;
+}
+
+}  // namespace
+
+bool fast_eligible(fx_solver S) {
+  if (fast_disabled() || S->A->nrow == 0) return false;
+  if (S->type == FX_SOLVER_CG)
+    return S->pc->kind == 0 || (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A == S->A);
+  if (S->type == FX_SOLVER_JACOBI) return S->pc->A == S->A;
+  return false;
+}
+
+int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
+  if (!S->fast) {
+    S->fast = new FastWs();
+    const size_t sb = S->type == FX_SOLVER_JACOBI ? sizeof(JacState) : sizeof(CgState);
+    int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
+                      spmv_grid(S->A->nblk));
+    if (rc) {
+      delete S->fast;
+      S->fast = nullptr;
+      return rc;
+    }
+  }
+  FastWs& W = *S->fast;
+  if (S->type == FX_SOLVER_JACOBI) return jacobi_run(S, W, (const double*)b->d, (double*)x->d, res);
+  CgRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
+                    S->ctx->stream};
+  return run.run((const double*)b->d, (double*)x->d, res, S->prm, FX_F64);
+}
+
+}  // namespace fx
+
+void fx_solver_s::release_fast() {
+  delete fast;
+  fast = nullptr;
+}
+
+using namespace fx;
+
+// ---------------------------------------------------------------------------
+// Mixed-precision iterative refinement
+// ---------------------------------------------------------------------------
+struct fx_ir_s {
+  fx_ctx ctx = nullptr;
+  int inner_type = FX_SOLVER_CG;
+  fx_csr A = nullptr;
+  fx_pc pc = nullptr;
+  fx_solver inner_s = nullptr;
+  fx_params outer{}, inner{};
+  float* r32 = nullptr;
+  float* d32 = nullptr;
+  double* dsc = nullptr;
+  unsigned* ctr = nullptr;
+  double* part = nullptr;
+  FastWs* W = nullptr;
+  ~fx_ir_s() {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    delete W;
+    fx_solver_destroy(inner_s);
+    fx_pc_destroy(pc);
+    (void)hipFree(r32);
+    (void)hipFree(d32);
+    (void)hipFree(dsc);
+    (void)hipFree(ctr);
+    (void)hipFree(part);
+  }
+};
+
+extern "C" {
+
+int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_params* outer,
+                 const fx_params* inner, fx_ir* out) {
+  clear_error();
+  FX_CHECK(ctx && A && outer && inner && out, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "IR needs a square matrix");
+  FX_CHECK(inner_type == FX_SOLVER_CG || inner_type == FX_SOLVER_GMRES, FX_ERROR_SOLVER_TYPE,
+           "IR inner solver must be CG or GMRES");
+  if (!A->va32 && A->has_values) FX_TRY(fx_csr_make_f32(A));
+  fx_ir ir = new fx_ir_s();
+  ir->ctx = ctx;
+  ir->inner_type = inner_type;
+  ir->A = A;
+  ir->outer = *outer;
+  ir->inner = *inner;
+  const uint32_t n = A->nrow;
+  auto fail = [&](int rc) {
+    delete ir;
+    return rc;
+  };
+  int rc = fx_pc_create_jacobi(ctx, A, weight, 1, &ir->pc);
+  if (!rc) rc = ensure_dinv32(ir->pc);
+  if (!rc) rc = fx_solver_create(ctx, inner_type, &ir->inner_s);
+  if (!rc) rc = fx_solver_setup(ir->inner_s, A, ir->pc, inner);
+  if (rc) return fail(rc);
+  const size_t nb = n ? n : 1;
+  const size_t np = 8 * ((size_t)spmv_grid(A->nblk) + kMaxRedBlocks);
+  if (hipMalloc(&ir->r32, sizeof(float) * nb) || hipMalloc(&ir->d32, sizeof(float) * nb) ||
+      hipMalloc(&ir->dsc, sizeof(double) * 4) || hipMalloc(&ir->ctr, sizeof(unsigned) * 4) ||
+      hipMalloc(&ir->part, sizeof(double) * np)) {
+    set_error("IR workspace allocation failed");
+    return fail(FX_ERROR_ALLOC_MEM);
+  }
+  if (hipMemsetAsync(ir->ctr, 0, sizeof(unsigned) * 4, ctx->stream)) return fail(FX_ERROR_DEVICE);
+  if (inner_type == FX_SOLVER_CG) {
+    ir->W = new FastWs();
+    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sizeof(CgState), spmv_grid(A->nblk));
+    if (rc) return fail(rc);
+  }
+  *out = ir;
+  return 0;
+}
+
+int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner_iters) {
+  clear_error();
+  FX_CHECK(ir && b && x && result, FX_ERROR_INPUT_PAR, "null argument");
+  fx_csr A = ir->A;
+  const uint32_t n = A->nrow;
+  FX_CHECK(b->n == n && x->n == n, FX_ERROR_NONMATCH_SIZE, "IR size mismatch");
+  FX_CHECK(b->dtype == FX_F64 && x->dtype == FX_F64, FX_ERROR_INPUT_PAR, "IR takes f64 b and x");
+  fx_ctx ctx = ir->ctx;
+  hipStream_t st = ctx->stream;
+  hipEvent_t e0, e1;
+  FX_HIP(hipEventCreate(&e0));
+  FX_HIP(hipEventCreate(&e1));
+  FX_HIP(hipEventRecord(e0, st));
+  uint32_t outer_it = 0, inner_total = 0;
+  double resAbs = 0.0, denAbs = 1.0;
+  int rc = 0, err = 0;
+  const fx_params& outer = ir->outer;
+  try {
+    for (;;) {
+      // r = b - A x (f64 values, Apply + XPAY order), r32 = (float) r, ||r||
+      rc = launch_spmv<double, double, ROW_AX, 1, 0u>(
+          A, st, XLoad<double>{(const double*)x->d}, BNone{},
+          EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part, ir->ctr);
+      if (rc) break;
+      double h = 0;
+      if (hipMemcpyAsync(&h, ir->dsc, sizeof(double), hipMemcpyDeviceToHost, st) ||
+          hipStreamSynchronize(st)) {
+        rc = FX_ERROR_DEVICE;
+        break;
+      }
+      resAbs = h;
+      if (outer_it == 0) denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+      if (resAbs / denAbs < outer.relTol || resAbs < outer.absTol) break;
+      if (outer_it >= outer.maxIter) {
+        err = FX_ERROR_SOLVER_MAXIT;
+        break;
+      }
+      ++outer_it;
+      // inner solve A d = r in f32 from d = 0
+      if (hipMemsetAsync(ir->d32, 0, sizeof(float) * n, st)) {
+        rc = FX_ERROR_DEVICE;
+        break;
+      }
+      fx_result res_in{};
+      if (ir->inner_type == FX_SOLVER_CG) {
+        CgRun<float> run{ir->inner_s, *ir->W, A, ir->pc->dinv32, make_prm(ir->inner), st};
+        const int irc = run.run(ir->r32, ir->d32, &res_in, ir->inner, FX_F32);
+        if (irc == FX_ERROR_DEVICE || irc == FX_ERROR_ALLOC_MEM || irc == FX_ERROR_INPUT_PAR) {
+          rc = irc;
+          break;
+        }
+      } else {
+        Ops o(ctx, A, ir->pc, FX_F32);
+        fx_vec vb = nullptr, vx = nullptr;
+        Ops::chk(fx_vec_wrap(ctx, n, FX_F32, ir->r32, &vb));
+        o.owned.push_back(vb);
+        Ops::chk(fx_vec_wrap(ctx, n, FX_F32, ir->d32, &vx));
+        o.owned.push_back(vx);
+        (void)ops_gmres(ir->inner_s, o, vb, vx, &res_in);
+      }
+      inner_total += res_in.numIter;
+      hipLaunchKernelGGL(k_add_f32_to_f64, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                         (double*)x->d, ir->d32, n);
+      if (hipGetLastError() != hipSuccess) {
+        rc = FX_ERROR_DEVICE;
+        break;
+      }
+    }
+  } catch (const OpsError& e) {
+    rc = e.code;
+  }
+  (void)hipEventRecord(e1, st);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  result->numIter = outer_it;
+  result->norm2 = resAbs;
+  result->normInf = 0.0;
+  result->solve_ms = ms;
+  result->retcode = rc ? rc : err;
+  if (inner_iters) *inner_iters = inner_total;
+  return result->retcode;
+}
+
+int fx_ir_destroy(fx_ir ir) {
+  delete ir;
+  return 0;
+}
+
+int fx_solve_ir(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_params* outer,
+                const fx_params* inner, fx_vec b, fx_vec x, fx_result* result,
+                uint32_t* inner_iters) {
+  fx_ir ir = nullptr;
+  FX_TRY(fx_ir_create(ctx, inner_type, A, weight, outer, inner, &ir));
+  const int rc = fx_ir_solve(ir, b, x, result, inner_iters);
+  fx_ir_destroy(ir);
+  return rc;
+}
+
+}  // extern "C"

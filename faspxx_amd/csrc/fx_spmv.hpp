@@ -1,0 +1,177 @@
+// fx_spmv.hpp -- the row-blocked CSR SpMV kernel (MAT::Apply /
+// MAT::Residual, src/MAT.cxx:373-476) with pluggable per-row epilogues and
+// an optional fused grid reduction.
+//
+// One workgroup (256 threads) per row block:
+//   * the block's row pointers are staged in LDS;
+//   * its column indices and values are read once, fully coalesced, and the
+//     gathered x[col] is staged beside each value in LDS;
+//   * rows are then reduced from LDS.  When the block has >= 129 rows each row
+//     is summed by one thread, left to right, in the reference's order
+//     (w = v0*x0, then w = fma(vk, xk, w)), so the result equals the
+//     reference's FMA-contracted build bit for bit.  Blocks with fewer, longer
+//     rows give each row a lane group of 2..64 lanes (butterfly reduction);
+//   * a row longer than the LDS stage gets a whole workgroup.
+// The epilogue receives (row, value) and may store, update other vectors and
+// add to per-thread reduction slots; the slots are then reduced over the grid
+// and the last workgroup runs the epilogue's finisher.
+#pragma once
+
+#include "fx_internal.hpp"
+
+namespace fx {
+
+enum { ROW_AX = 0, ROW_B_MINUS_AX = 1 };
+
+template <class MT, class VT>
+struct SpmvIn {
+  const uint32_t* __restrict__ rp;
+  const uint32_t* __restrict__ ci;
+  const MT* __restrict__ va;  // null: structure-only matrix (values are 1)
+  const uint32_t* __restrict__ blk;
+  uint32_t nblk;
+};
+
+template <class MT, class VT>
+inline SpmvIn<MT, VT> spmv_in(fx_csr A);
+
+template <>
+inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
+  return {A->rp, A->ci, A->has_values ? A->va : nullptr, A->blk, A->nblk};
+}
+template <>
+inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
+  return {A->rp, A->ci, A->has_values ? A->va32 : nullptr, A->blk, A->nblk};
+}
+
+// LDS footprint: row pointers + staged (value, x) pairs.
+template <class MT, class VT>
+struct SpmvLds {
+  uint32_t rp[kBlockRows + 1];
+  MT va[kBlockNnz];
+  VT x[kBlockNnz];
+};
+
+// Threads per row for a block of nr rows: largest power of two <= 256/nr,
+// capped at one wave.
+__device__ __forceinline__ int threads_per_row(uint32_t nr) {
+  if (nr > 128) return 1;
+  const uint32_t q = kThreads / (nr ? nr : 1);
+  int t = 1;
+  while ((uint32_t)(t * 2) <= q && t < 64) t *= 2;
+  return t;
+}
+
+// Generic kernel.  X is a functor giving the gathered vector entry x_j
+// (a plain load, or a value formed on the fly); B gives b_i for
+// ROW_B_MINUS_AX.  Epi must provide:
+//   bool skip() const                      uniform early exit (solver stop flag)
+//   void row(uint32_t r, double v, double (&acc)[NRED])
+//   void fin(double (&sum)[NRED])          last workgroup, all threads
+template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
+__global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf, BF bf, Epi epi,
+                                                   double* partials, unsigned* counter) {
+  if (epi.skip()) return;
+  __shared__ SpmvLds<MT, VT> lds;
+  double acc[NRED > 0 ? NRED : 1];
+#pragma unroll
+  for (int k = 0; k < (NRED > 0 ? NRED : 1); ++k) acc[k] = 0.0;
+  const unsigned bid = xcd_block();
+  const int tid = threadIdx.x;
+  if (bid < in.nblk) {
+    const uint32_t r0 = in.blk[bid], r1 = in.blk[bid + 1];
+    const uint32_t k0 = in.rp[r0], k1 = in.rp[r1];
+    const uint32_t nr = r1 - r0, nz = k1 - k0;
+    if (nz > kBlockNnz) {
+      // a single long row: strided partial sums, then a workgroup reduction
+      double s = 0.0;
+      for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
+        const double a = in.va ? (double)in.va[k] : 1.0;
+        s = fma(a, (double)xf(in.ci[k]), s);
+      }
+      double t[1] = {s};
+      block_reduce<1>(t);
+      if (tid == 0) {
+        const double v = (MODE == ROW_AX) ? t[0] : (double)bf(r0) - t[0];
+        epi.row(r0, v, acc);
+      }
+    } else {
+      for (uint32_t i = tid; i <= nr; i += kThreads) lds.rp[i] = in.rp[r0 + i] - k0;
+      for (uint32_t k = tid; k < nz; k += kThreads) {
+        const uint32_t c = in.ci[k0 + k];
+        lds.va[k] = in.va ? in.va[k0 + k] : (MT)1;
+        lds.x[k] = xf(c);
+      }
+      __syncthreads();
+      const int tpr = threads_per_row(nr);
+      if (tpr == 1) {
+        if ((uint32_t)tid < nr) {
+          const uint32_t s = lds.rp[tid], e = lds.rp[tid + 1];
+          double t;
+          if (s == e) {
+            t = (MODE == ROW_AX) ? 0.0 : (double)bf(r0 + tid);
+          } else if (MODE == ROW_AX) {
+            t = (double)lds.va[s] * (double)lds.x[s];
+            for (uint32_t k = s + 1; k < e; ++k) t = fma((double)lds.va[k], (double)lds.x[k], t);
+          } else {
+            t = fma(-(double)lds.va[s], (double)lds.x[s], (double)bf(r0 + tid));
+            for (uint32_t k = s + 1; k < e; ++k) t = fma(-(double)lds.va[k], (double)lds.x[k], t);
+          }
+          epi.row(r0 + tid, t, acc);
+        }
+      } else {
+        const int grp = tid / tpr, lane = tid % tpr, ngrp = kThreads / tpr;
+        for (uint32_t rr = grp; rr < nr; rr += ngrp) {
+          const uint32_t s = lds.rp[rr], e = lds.rp[rr + 1];
+          double t = 0.0;
+          for (uint32_t k = s + lane; k < e; k += tpr)
+            t = fma((double)lds.va[k], (double)lds.x[k], t);
+          t = group_sum(t, tpr);
+          if (lane == 0) {
+            const double v = (MODE == ROW_AX) ? t : (double)bf(r0 + rr) - t;
+            epi.row(r0 + rr, v, acc);
+          }
+        }
+      }
+    }
+  }
+  if constexpr (NRED > 0) {
+    if (grid_reduce<NRED, MAXMASK>(acc, partials, counter)) epi.fin(acc);
+  }
+}
+
+// Functors for the common cases.
+template <class VT>
+struct XLoad {
+  const VT* __restrict__ x;
+  __device__ __forceinline__ VT operator()(uint32_t j) const { return x[j]; }
+};
+struct BNone {
+  __device__ __forceinline__ double operator()(uint32_t) const { return 0.0; }
+};
+template <class VT>
+struct BLoad {
+  const VT* __restrict__ b;
+  __device__ __forceinline__ double operator()(uint32_t i) const { return (double)b[i]; }
+};
+
+// y = v (store) -- MAT::Apply
+template <class VT>
+struct EpiStore {
+  VT* __restrict__ y;
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t r, double v, double (&)[1]) const { y[r] = (VT)v; }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
+inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
+                       unsigned* counter) {
+  const unsigned g = spmv_grid(A->nblk);
+  hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g), dim3(kThreads), 0,
+                     st, spmv_in<MT, VT>(A), xf, bf, epi, partials, counter);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fx

@@ -1,0 +1,306 @@
+"""ctypes access to the oracle (TEST INFRASTRUCTURE ONLY).
+
+``Oracle`` wraps liboracle.so, the CPU restatement of the reference path
+(fx_oracle.cpp).  ``Reference`` wraps _ref/libfaspxx_ref.so, the reference's
+own sources compiled by oracle/Makefile (present only where the reference
+checkout was available at build time).  Only tests/, the smoke check and the
+benchmark's CPU-baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libfaspxx_ref.so"
+
+CG, BICGSTAB, GMRES, FGMRES, JACOBI = 1, 2, 4, 5, 11
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference, when its sources exist)."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", str(HERE), target], check=True)
+
+
+class _Csr(C.Structure):
+    _fields_ = [("nrow", C.c_uint32), ("mcol", C.c_uint32), ("nnz", C.c_uint32),
+                ("rp", C.c_void_p), ("ci", C.c_void_p), ("va", C.c_void_p), ("dp", C.c_void_p)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("maxIter", C.c_uint32), ("minIter", C.c_uint32), ("savIter", C.c_uint32),
+                ("restart", C.c_uint32), ("relTol", C.c_double), ("absTol", C.c_double),
+                ("maxRestart", C.c_uint32), ("minRestart", C.c_uint32), ("leftPrecond", C.c_int)]
+
+
+class _Pc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("iters", C.c_uint32), ("dinv", C.c_void_p)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("retcode", C.c_int), ("numIter", C.c_uint32), ("norm2", C.c_double),
+                ("normInf", C.c_double)]
+
+
+class _RefParams(C.Structure):
+    _fields_ = [("maxIter", C.c_uint32), ("minIter", C.c_uint32), ("savIter", C.c_uint32),
+                ("restart", C.c_uint32), ("relTol", C.c_double), ("absTol", C.c_double)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+DEFAULTS = dict(maxIter=1, minIter=0, savIter=100000, restart=30, relTol=1e-6, absTol=1e-8,
+                maxRestart=30, minRestart=10, leftPrecond=0)
+
+
+class Matrix:
+    """Holds CSRx arrays alive for ctypes calls."""
+
+    def __init__(self, nrow, mcol, rp, ci, va=None, dp=None):
+        self.nrow, self.mcol = int(nrow), int(mcol)
+        self.rp, self.ci = _u32(rp), _u32(ci)
+        self.va = None if va is None else _f64(va)
+        self.dp = None if dp is None else _u32(dp)
+        self.nnz = int(self.ci.shape[0])
+        self.c = _Csr(self.nrow, self.mcol, self.nnz, _p(self.rp), _p(self.ci), _p(self.va),
+                      _p(self.dp))
+
+    @classmethod
+    def of(cls, A) -> "Matrix":
+        return cls(A.nrow, A.mcol, A.row_ptr, A.col_ind, A.values, A.diag_ptr)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not pathlib.Path(path).exists():
+            build(ref=False)
+        L = C.CDLL(str(path))
+        cs = C.POINTER(_Csr)
+        d = C.c_double
+        u = C.c_uint32
+        v = C.c_void_p
+        sig = {
+            "orc_dot": (d, [u, v, v]), "orc_norm2": (d, [u, v]), "orc_norminf": (d, [u, v]),
+            "orc_axpy": (None, [u, d, v, v]), "orc_xpay": (None, [u, d, v, v]),
+            "orc_axpby": (None, [u, d, d, v, v]), "orc_waxpby": (None, [u, d, v, d, v, v]),
+            "orc_pmul": (None, [u, v, v]), "orc_scale": (None, [u, d, v]),
+            "orc_add": (None, [u, v, v]),
+            "orc_spmv": (None, [cs, v, v]), "orc_residual": (None, [cs, v, v, v]),
+            "orc_form_diagptr": (None, [u, v, v, v]), "orc_get_diag": (None, [cs, v]),
+            "orc_jacobi_dinv": (None, [cs, d, v]), "orc_transpose": (None, [cs, v, v, v, v]),
+            "orc_check_csrx": (C.c_int, [cs]),
+            "orc_solve": (C.c_int, [C.c_int, cs, C.POINTER(_Pc), C.POINTER(_Params), v, v,
+                                    C.POINTER(_Result), v, u]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        self.L = L
+
+    # VEC
+    def dot(self, a, b):
+        a, b = _f64(a), _f64(b)
+        return self.L.orc_dot(a.shape[0], _p(a), _p(b))
+
+    def norm2(self, a):
+        a = _f64(a)
+        return self.L.orc_norm2(a.shape[0], _p(a))
+
+    def norminf(self, a):
+        a = _f64(a)
+        return self.L.orc_norminf(a.shape[0], _p(a))
+
+    def axpy(self, a, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.orc_axpy(x.shape[0], a, _p(x), _p(y))
+        return y
+
+    def xpay(self, a, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.orc_xpay(x.shape[0], a, _p(x), _p(y))
+        return y
+
+    def axpby(self, a, b, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.orc_axpby(x.shape[0], a, b, _p(x), _p(y))
+        return y
+
+    def waxpby(self, a, x, b, y):
+        x, y = _f64(x), _f64(y)
+        w = np.empty_like(x)
+        self.L.orc_waxpby(x.shape[0], a, _p(x), b, _p(y), _p(w))
+        return w
+
+    # MAT
+    def spmv(self, M: Matrix, v):
+        v = _f64(v)
+        w = np.empty(M.nrow)
+        self.L.orc_spmv(C.byref(M.c), _p(v), _p(w))
+        return w
+
+    def residual(self, M: Matrix, b, x):
+        b, x = _f64(b), _f64(x)
+        r = np.empty(M.nrow)
+        self.L.orc_residual(C.byref(M.c), _p(b), _p(x), _p(r))
+        return r
+
+    def form_diagptr(self, nrow, rp, ci):
+        rp, ci = _u32(rp), _u32(ci)
+        dp = np.zeros(nrow, np.uint32)
+        self.L.orc_form_diagptr(nrow, _p(rp), _p(ci), _p(dp))
+        return dp
+
+    def jacobi_dinv(self, M: Matrix, weight):
+        d = np.empty(M.nrow)
+        self.L.orc_jacobi_dinv(C.byref(M.c), weight, _p(d))
+        return d
+
+    def transpose(self, M: Matrix):
+        rp = np.zeros(M.mcol + 1, np.uint32)
+        ci = np.zeros(M.nnz, np.uint32)
+        va = np.zeros(M.nnz) if M.va is not None else None
+        dp = np.zeros(M.mcol, np.uint32)
+        self.L.orc_transpose(C.byref(M.c), _p(rp), _p(ci), _p(va), _p(dp))
+        return rp, ci, va, dp
+
+    def check_csrx(self, M: Matrix) -> int:
+        return self.L.orc_check_csrx(C.byref(M.c))
+
+    # solvers
+    def solve(self, kind, M: Matrix, b, x0, pc="identity", weight=1.0, pc_iters=1,
+              hist_cap=0, **prm):
+        P = dict(DEFAULTS)
+        P.update(prm)
+        params = _Params(**P)
+        b = _f64(b)
+        x = _f64(x0).copy()
+        dinv = None
+        if pc == "jacobi" or kind == JACOBI:
+            dinv = self.jacobi_dinv(M, weight)
+        pcs = _Pc(1 if dinv is not None else 0, pc_iters, _p(dinv))
+        res = _Result()
+        hist = np.zeros(hist_cap) if hist_cap else None
+        self.L.orc_solve(kind, C.byref(M.c), C.byref(pcs), C.byref(params), _p(b), _p(x),
+                         C.byref(res), _p(hist), hist_cap)
+        out = {"retcode": res.retcode, "numIter": res.numIter, "norm2": res.norm2,
+               "normInf": res.normInf, "x": x}
+        if hist is not None:
+            out["hist"] = hist
+        return out
+
+
+class Reference:
+    """The compiled reference (oracle/_ref).  GMRES / FGMRES are not exposed:
+    their back-substitution loops do not terminate (src/GMRES.cxx:214)."""
+
+    def __init__(self, path=REF_SO):
+        if not pathlib.Path(path).exists():
+            raise FileNotFoundError(path)
+        L = C.CDLL(str(path))
+        d, u, v = C.c_double, C.c_uint32, C.c_void_p
+        sig = {
+            "ref_spmv": (None, [u, u, u, v, v, v, v, v, v]),
+            "ref_residual": (None, [u, u, u, v, v, v, v, v, v, v]),
+            "ref_transpose": (None, [u, u, u, v, v, v, v, v, v, v]),
+            "ref_check_csrx": (C.c_int, [u, u, u, v, v, v, v]),
+            "ref_dot": (d, [u, v, v]), "ref_norm2": (d, [u, v]), "ref_norminf": (d, [u, v]),
+            "ref_axpy": (None, [u, d, v, v]), "ref_xpay": (None, [u, d, v, v]),
+            "ref_axpby": (None, [u, d, d, v, v]), "ref_waxpby": (None, [u, d, v, d, v, v]),
+            "ref_solve": (C.c_int, [C.c_int, u, u, v, v, v, v, C.c_int, u, d,
+                                    C.POINTER(_RefParams), v, v, C.POINTER(_Result)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        self.L = L
+
+    @staticmethod
+    def available() -> bool:
+        return REF_SO.exists()
+
+    def _m(self, M: Matrix):
+        return (M.nrow, M.mcol, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), _p(M.dp))
+
+    def spmv(self, M: Matrix, v):
+        v = _f64(v)
+        w = np.empty(M.nrow)
+        self.L.ref_spmv(*self._m(M), _p(v), _p(w))
+        return w
+
+    def residual(self, M: Matrix, b, x):
+        b, x = _f64(b), _f64(x)
+        r = np.empty(M.nrow)
+        self.L.ref_residual(*self._m(M), _p(b), _p(x), _p(r))
+        return r
+
+    def transpose(self, M: Matrix):
+        rp = np.zeros(M.mcol + 1, np.uint32)
+        ci = np.zeros(M.nnz, np.uint32)
+        va = np.zeros(M.nnz) if M.va is not None else None
+        self.L.ref_transpose(*self._m(M), _p(rp), _p(ci), _p(va))
+        return rp, ci, va
+
+    def check_csrx(self, M: Matrix) -> int:
+        return self.L.ref_check_csrx(*self._m(M))
+
+    def dot(self, a, b):
+        a, b = _f64(a), _f64(b)
+        return self.L.ref_dot(a.shape[0], _p(a), _p(b))
+
+    def norm2(self, a):
+        a = _f64(a)
+        return self.L.ref_norm2(a.shape[0], _p(a))
+
+    def norminf(self, a):
+        a = _f64(a)
+        return self.L.ref_norminf(a.shape[0], _p(a))
+
+    def axpy(self, a, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.ref_axpy(x.shape[0], a, _p(x), _p(y))
+        return y
+
+    def xpay(self, a, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.ref_xpay(x.shape[0], a, _p(x), _p(y))
+        return y
+
+    def axpby(self, a, b, x, y):
+        x, y = _f64(x), _f64(y).copy()
+        self.L.ref_axpby(x.shape[0], a, b, _p(x), _p(y))
+        return y
+
+    def waxpby(self, a, x, b, y):
+        x, y = _f64(x), _f64(y)
+        w = np.empty_like(x)
+        self.L.ref_waxpby(x.shape[0], a, _p(x), b, _p(y), _p(w))
+        return w
+
+    def solve(self, kind, M: Matrix, b, x0, pc="identity", weight=1.0, pc_iters=1, **prm):
+        P = dict(DEFAULTS)
+        P.update(prm)
+        params = _RefParams(P["maxIter"], P["minIter"], P["savIter"], P["restart"],
+                            P["relTol"], P["absTol"])
+        b = _f64(b)
+        x = _f64(x0).copy()
+        res = _Result()
+        self.L.ref_solve(kind, M.nrow, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), _p(M.dp),
+                         1 if pc == "jacobi" else 0, pc_iters, weight, C.byref(params), _p(b),
+                         _p(x), C.byref(res))
+        return {"retcode": res.retcode, "numIter": res.numIter, "norm2": res.norm2,
+                "normInf": res.normInf, "x": x}

@@ -1,0 +1,196 @@
+"""GPU solver parity against the oracle (CPU restatement of src/CG.cxx,
+src/BiCGStab.cxx, src/GMRES.cxx, src/FGMRES.cxx, src/Iter.cxx), through the
+C ABI.  Bars: iteration counts within +-2 at the same relTol; solutions and
+final residual norms within the stated relative tolerances; weighted-Jacobi
+sweeps bit-exact (their updates contain no reduction)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from faspxx_amd import fx
+from oracle import BICGSTAB, CG, FGMRES, GMRES, JACOBI, Matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, a, dt=fx.F64):
+    return fx.Vec.from_numpy(ctx, np.asarray(a), dt)
+
+
+def gpu_solve(ctx, pr, kind, pc="identity", weight=1.0, pc_iters=1, x0=None, force_ops=False,
+              **prm):
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "jacobi" if (pc == "jacobi" or kind == JACOBI) else "identity", A,
+                   weight, pc_iters)
+    s = fx.Solver(ctx, kind)
+    s.Setup(A, P, fx.params(**prm))
+    x = dev(ctx, np.zeros(pr.A.nrow) if x0 is None else x0)
+    old = os.environ.get("FASPXX_FX_NO_FAST")
+    if force_ops:
+        os.environ["FASPXX_FX_NO_FAST"] = "1"
+    try:
+        res = s.Solve(dev(ctx, pr.b), x)
+    finally:
+        if force_ops:
+            if old is None:
+                del os.environ["FASPXX_FX_NO_FAST"]
+            else:
+                os.environ["FASPXX_FX_NO_FAST"] = old
+    return res, x.numpy(), s.history()
+
+
+PRM = dict(maxIter=3000, relTol=1e-8, absTol=0.0)
+
+CG_CASES = [(1, None, "identity", 1.0), (1, None, "jacobi", 0.8), (2, 32, "jacobi", 1.0),
+            (5, 64, "jacobi", 1.0), (4, 16384, "jacobi", 1.0)]
+
+
+@pytest.mark.parametrize("force_ops", [False, True])
+@pytest.mark.parametrize("cid,scale,pc,w", CG_CASES)
+def test_cg_parity(ctx, oracle, problems, cid, scale, pc, w, force_ops):
+    pr = problems(cid, scale)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc=pc, weight=w, hist_cap=4000, **PRM)
+    res, x, hist = gpu_solve(ctx, pr, fx.CG, pc, w, force_ops=force_ops, **PRM)
+    assert res.retcode == o["retcode"] == 0
+    assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+    assert rel_err(x, o["x"]) < 1e-7
+    true_res = np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b)
+    assert true_res < 1e-8
+    assert abs(res.norm2 - o["norm2"]) <= 1e-3 * o["norm2"] + 1e-14
+    k = min(len(hist), 30)
+    assert k > 10 and rel_err(hist[:k], o["hist"][:k]) < 1e-10
+
+
+def test_cg_x0_and_defaults(ctx, oracle, problems):
+    pr = problems(1, 64)
+    M = Matrix.of(pr.A)
+    x0 = np.ones(M.nrow)
+    for prm in [dict(), dict(maxIter=25), dict(maxIter=25, minIter=5, relTol=1e-3)]:
+        o = oracle.solve(CG, M, pr.b, x0, pc="jacobi", **prm)
+        res, x, _ = gpu_solve(ctx, pr, fx.CG, "jacobi", x0=x0, **prm)
+        assert res.numIter == o["numIter"] and res.retcode == o["retcode"], prm
+        assert rel_err(x, o["x"]) < 1e-10
+
+
+def test_cg_zero_rhs_and_maxit(ctx, problems):
+    pr = problems(1, 32)
+
+    class Z:
+        A = pr.A
+        b = np.zeros(pr.A.nrow)
+    res, x, _ = gpu_solve(ctx, Z, fx.CG, maxIter=100, relTol=1e-8, absTol=0.0)
+    assert res.numIter == 0 and np.all(x == 0)
+    res, x, _ = gpu_solve(ctx, pr, fx.CG, maxIter=7, relTol=1e-12, absTol=0.0)
+    assert res.numIter == 7
+
+
+def test_cg_tolsmall_path(ctx, oracle, problems):
+    """relTol below attainable accuracy exercises the host-side check III."""
+    pr = problems(1, 16)
+    M = Matrix.of(pr.A)
+    prm = dict(maxIter=2000, relTol=1e-19, absTol=0.0, restart=2)
+    o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), **prm)
+    res, x, _ = gpu_solve(ctx, pr, fx.CG, **prm)
+    assert res.retcode == o["retcode"]
+    assert rel_err(x, o["x"]) < 1e-10
+
+
+@pytest.mark.parametrize("force_ops", [False, True])
+def test_jacobi_sweeps_bitexact(ctx, oracle, problems, force_ops):
+    pr = problems(2, 32)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(JACOBI, M, pr.b, np.zeros(M.nrow), weight=2 / 3, maxIter=50, relTol=1e-30,
+                     absTol=0.0, hist_cap=60)
+    res, x, hist = gpu_solve(ctx, pr, fx.JACOBI, weight=2 / 3, maxIter=50, relTol=1e-30,
+                             absTol=0.0, force_ops=force_ops)
+    assert res.numIter == o["numIter"] == 50
+    assert np.array_equal(x, o["x"])
+    assert abs(res.norm2 - o["norm2"]) <= 1e-13 * o["norm2"]
+    assert rel_err(hist, o["hist"][:50]) < 1e-13
+
+
+def test_jacobi_converges_unit(ctx):
+    """UnitTestsJacobi 3x3 (utest/src/UnitTestsJacobi.cxx:85-157)."""
+    class PR:
+        pass
+    from faspxx_amd.problems import CSR
+    PR.A = CSR(3, 3, np.array([0, 3, 6, 9], np.uint32), np.array([0, 1, 2] * 3, np.uint32),
+               np.array([4, 2, 1, 3, 8, 2, 2, 1, 9], np.float64), np.array([0, 4, 8], np.uint32))
+    PR.b = np.ones(3)
+    for w in (1.0, 0.5):
+        res, x, _ = gpu_solve(ctx, PR, fx.JACOBI, weight=w, maxIter=100, relTol=1e-9)
+        assert np.all(np.abs(x - [0.221719, 0.027149, 0.058824]) < 1e-5)
+
+
+@pytest.mark.parametrize("pc", ["jacobi", "identity"])
+def test_bicgstab_parity(ctx, oracle, problems, pc):
+    pr = problems(3, 128)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(BICGSTAB, M, pr.b, np.zeros(M.nrow), pc=pc, **PRM)
+    res, x, _ = gpu_solve(ctx, pr, fx.BICGSTAB, pc, **PRM)
+    assert res.retcode == o["retcode"] == 0
+    assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+    assert rel_err(x, o["x"]) < 1e-6
+
+
+@pytest.mark.parametrize("left", [0, 1])
+@pytest.mark.parametrize("pc", ["jacobi", "identity"])
+def test_gmres_parity(ctx, oracle, problems, pc, left):
+    pr = problems(3, 64)
+    M = Matrix.of(pr.A)
+    prm = dict(maxIter=4000, relTol=1e-8, absTol=0.0, leftPrecond=left)
+    o = oracle.solve(GMRES, M, pr.b, np.zeros(M.nrow), pc=pc, **prm)
+    res, x, _ = gpu_solve(ctx, pr, fx.GMRES, pc, **prm)
+    assert res.retcode == o["retcode"] == 0
+    assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+    assert rel_err(x, o["x"]) < 1e-6
+
+
+def test_fgmres_parity(ctx, oracle, problems):
+    pr = problems(3, 48)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(FGMRES, M, pr.b, np.zeros(M.nrow), pc="identity", **PRM)
+    res, x, _ = gpu_solve(ctx, pr, fx.FGMRES, "identity", **PRM)
+    assert abs(res.numIter - o["numIter"]) <= 2
+    assert rel_err(x, o["x"]) < 1e-6
+    # Jacobi with the reference's stale initial guess (src/FGMRES.cxx:158-159):
+    # same trajectory for a fixed iteration budget
+    prm = dict(maxIter=200, relTol=1e-12, absTol=0.0)
+    o = oracle.solve(FGMRES, M, pr.b, np.zeros(M.nrow), pc="jacobi", **prm)
+    res, x, _ = gpu_solve(ctx, pr, fx.FGMRES, "jacobi", **prm)
+    assert res.numIter == o["numIter"] == 200
+    assert abs(res.norm2 - o["norm2"]) <= 1e-6 * o["norm2"]
+
+
+def test_krylov_front_end(ctx, problems):
+    pr = problems(1, 32)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "identity")
+    for name, ok in [("CG", True), ("BiCGStab", True), ("gmres", True), ("FGMRES", True),
+                     ("nonsense", True)]:
+        x = fx.Vec(ctx, pr.A.nrow)
+        res = fx.krylov(ctx, A, dev(ctx, pr.b), x, P, name, fx.params(maxIter=2000, relTol=1e-8,
+                                                                      absTol=0.0))
+        assert res.retcode == 0
+        assert np.linalg.norm(pr.b - pr.A.matvec(x.numpy())) / np.linalg.norm(pr.b) < 1e-7
+
+
+@pytest.mark.parametrize("cid,scale", [(2, 32), (5, 64), (4, 16384)])
+def test_ir_mixed_precision(ctx, oracle, problems, cid, scale):
+    pr = problems(cid, scale)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    x = fx.Vec(ctx, pr.A.nrow)
+    res, inner = fx.solve_ir(ctx, A, dev(ctx, pr.b), x, 1.0,
+                             fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
+                             fx.params(maxIter=5000, relTol=1e-4, absTol=0.0))
+    assert res.retcode == 0
+    xr = x.numpy()
+    true_res = np.linalg.norm(pr.b - pr.A.matvec(xr)) / np.linalg.norm(pr.b)
+    assert true_res < 1e-10
+    M = Matrix.of(pr.A)
+    o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc="jacobi", maxIter=20000, relTol=1e-12,
+                     absTol=0.0)
+    assert rel_err(xr, o["x"]) < 1e-6
