@@ -6,11 +6,11 @@
 //   * the block's row pointers are staged in LDS;
 //   * its column indices and values are read once, fully coalesced, and the
 //     gathered x[col] is staged beside each value in LDS;
-//   * rows are then reduced from LDS.  When the block has >= 129 rows each row
-//     is summed by one thread, left to right, in the reference's order
-//     (w = v0*x0, then w = fma(vk, xk, w)), so the result equals the
-//     reference's FMA-contracted build bit for bit.  Blocks with fewer, longer
-//     rows give each row a lane group of 2..64 lanes (butterfly reduction);
+//   * rows are then reduced from LDS.  While the block's rows average at most
+//     32 entries each row is summed by one thread, left to right, in the
+//     reference's order (w = v0*x0, then w = fma(vk, xk, w)), so the result
+//     equals the reference's FMA-contracted build bit for bit.  Blocks of
+//     longer rows give each row a lane group of 2..64 lanes (butterfly sum);
 //   * a row longer than the LDS stage gets a whole workgroup.
 // The epilogue receives (row, value) and may store, update other vectors and
 // add to per-thread reduction slots; the slots are then reduced over the grid
@@ -52,11 +52,17 @@ struct SpmvLds {
   VT x[kBlockNnz];
 };
 
-// Threads per row for a block of nr rows: largest power of two <= 256/nr,
+// Threads per row for a block of nr rows holding nz entries: one thread per
+// row (the reference's summation order) while rows average at most
+// kSeqRowLen entries; longer rows get the largest power-of-two lane group
+// that both fits 256/nr and leaves each lane at least kSeqRowLen/2 entries,
 // capped at one wave.
-__device__ __forceinline__ int threads_per_row(uint32_t nr) {
-  if (nr > 128) return 1;
-  const uint32_t q = kThreads / (nr ? nr : 1);
+constexpr uint32_t kSeqRowLen = 32;
+__device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
+  if (nr == 0 || nz <= kSeqRowLen * nr) return 1;
+  const uint32_t by_rows = kThreads / nr;
+  const uint32_t by_len = (2 * nz) / (kSeqRowLen * nr);
+  const uint32_t q = by_rows < by_len ? by_rows : by_len;
   int t = 1;
   while ((uint32_t)(t * 2) <= q && t < 64) t *= 2;
   return t;
@@ -103,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf, BF 
         lds.x[k] = xf(c);
       }
       __syncthreads();
-      const int tpr = threads_per_row(nr);
+      const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {
         if ((uint32_t)tid < nr) {
           const uint32_t s = lds.rp[tid], e = lds.rp[tid + 1];

@@ -9,9 +9,11 @@ to the HIP library; there is no CPU fallback: if the library is missing,
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -55,6 +57,15 @@ class FxError(RuntimeError):
 
 
 _lib = None
+_exiting = False
+
+
+@atexit.register
+def _mark_exit():
+    # Objects still alive at interpreter exit are not freed one by one: the
+    # HIP runtime may already be shutting down, and the process is ending.
+    global _exiting
+    _exiting = True
 
 _P = C.c_void_p
 _SIGS = {
@@ -174,6 +185,10 @@ class Context:
             _chk(lib.fx_ctx_create_on_stream(device, C.c_void_p(stream), C.byref(h)))
         self.h = h
         self.device = device
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def sync(self):
         _chk(load().fx_ctx_sync(self.h))
@@ -183,15 +198,20 @@ class Context:
         return load().fx_ctx_stream(self.h) or 0
 
     def close(self):
+        """Destroy every live object created on this context, then the context
+        (the C ABI requires objects to be destroyed before their context)."""
         if self.h:
+            for child in list(self._children):
+                child.close()
             load().fx_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class Vec:
@@ -205,6 +225,7 @@ class Vec:
         else:
             _chk(lib.fx_vec_wrap(ctx.h, n, dtype, C.c_void_p(_wrap_ptr), C.byref(h)))
         self.h, self.ctx, self.n, self.dtype = h, ctx, n, dtype
+        ctx._adopt(self)
 
     @classmethod
     def from_numpy(cls, ctx: Context, a: np.ndarray, dtype=None) -> "Vec":
@@ -285,10 +306,11 @@ class Vec:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class Csr:
@@ -307,6 +329,7 @@ class Csr:
                                None if self._va is None else _ptr(self._va),
                                None if self._dp is None else _ptr(self._dp), C.byref(h)))
         self.h, self.ctx = h, ctx
+        ctx._adopt(self)
         self.nrow, self.mcol, self.nnz = nrow, mcol, nnz
         # host copies are not needed after upload
         self._rp = self._ci = self._va = self._dp = None
@@ -319,6 +342,7 @@ class Csr:
     def _from_handle(cls, ctx: Context, h) -> "Csr":
         obj = cls.__new__(cls)
         obj.h, obj.ctx = h, ctx
+        ctx._adopt(obj)
         nr, mc, nz = C.c_uint32(), C.c_uint32(), C.c_uint32()
         _chk(load().fx_csr_info(h, C.byref(nr), C.byref(mc), C.byref(nz)))
         obj.nrow, obj.mcol, obj.nnz = nr.value, mc.value, nz.value
@@ -362,10 +386,11 @@ class Csr:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class Precond:
@@ -380,6 +405,7 @@ class Precond:
         else:
             raise ValueError(kind)
         self.h, self.ctx, self.A = h, ctx, A
+        ctx._adopt(self)
 
     def Solve(self, b: Vec, x: Vec):
         _chk(load().fx_pc_apply(self.ctx.h, self.h, b.h, x.h))
@@ -390,10 +416,11 @@ class Precond:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class Solver:
@@ -402,6 +429,7 @@ class Solver:
         _chk(load().fx_solver_create(ctx.h, kind, C.byref(h)))
         self.h, self.ctx, self.kind = h, ctx, kind
         self._keep = ()
+        ctx._adopt(self)
 
     def Setup(self, A: Csr, pc: Precond, prm: Params):
         self._keep = (A, pc)
@@ -430,10 +458,11 @@ class Solver:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 def krylov(ctx: Context, A: Csr, b: Vec, x: Vec, pc: Precond, name: str, prm: Params) -> Result:
@@ -464,6 +493,7 @@ class IR:
         _chk(load().fx_ir_create(ctx.h, inner_kind, A.h, weight, C.byref(outer), C.byref(inner),
                                  C.byref(h)))
         self.h, self.ctx, self._keep = h, ctx, (A,)
+        ctx._adopt(self)
 
     def Solve(self, b: Vec, x: Vec) -> tuple[Result, int]:
         res = Result()
@@ -479,7 +509,8 @@ class IR:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
