@@ -56,7 +56,10 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    stream = torch.cuda.current_stream()
+    # a dedicated stream (default priority): the legacy null stream cannot
+    # be captured into the solvers' replay graphs
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     ctx = fx.Context(0, stream=stream.cuda_stream)
 
     t0 = time.time()
@@ -77,7 +80,10 @@ def main() -> int:
         return torch.cuda.Event(enable_timing=True)
 
     # ---- standalone SpMV, Infinity Cache flushed between reps ------------
-    flush = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    # Flush by READING 512 MB (twice the 256 MB Infinity Cache): a write-based
+    # flush would leave dirty lines whose write-back lands inside the timed SpMV.
+    flush = torch.ones(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    flush_sink = torch.empty((), dtype=torch.float32, device=dev)
     res_spmv = {}
     for dt, s in ((fx.F64, 8), (fx.F32, 4)):
         xv = fx.Vec.from_numpy(ctx, np.random.default_rng(1).uniform(-1, 1, n),
@@ -87,7 +93,7 @@ def main() -> int:
             A.Apply(xv, yv)
         times = []
         for i in range(args.spmv_reps):
-            flush.fill_(float(i))
+            torch.sum(flush, dim=0, out=flush_sink)
             e0, e1 = ev(), ev()
             e0.record(stream)
             A.Apply(xv, yv)

@@ -87,13 +87,13 @@ __global__ __launch_bounds__(kThreads) void k_convert(const S* __restrict__ s, D
 
 // Reductions: kind 0 dot(x,y), 1 sum x^2 (then sqrt), 2 max |x|.  Optional
 // fused update y += a x before the reduction (kinds 0/1 with `upd`).
-// Products are formed in double; sums accumulate in double with fma.
+// Products are formed in double; sums accumulate in double with fma.  Each
+// workgroup writes one partial; k_finish combines them.
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_reduce(int kind, int upd, double a,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      const T* __restrict__ z, uint32_t n,
-                                                     double* partials, unsigned* counter,
-                                                     double* out) {
+                                                     double* partials) {
   double acc[1] = {0.0};
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
@@ -109,9 +109,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int kind, int upd, double a
       acc[0] = fma(u, kind == 0 ? (double)y[i] : u, acc[0]);
     }
   }
-  bool last = (kind == 2) ? grid_reduce<1, 1u>(acc, partials, counter)
-                          : grid_reduce<1, 0u>(acc, partials, counter);
-  if (last && threadIdx.x == 0) *out = (kind == 1) ? sqrt(acc[0]) : acc[0];
+  if (kind == 2)
+    block_partials<1, 1u>(acc, partials);
+  else
+    block_partials<1, 0u>(acc, partials);
 }
 
 template <class T>
@@ -144,15 +145,14 @@ int reduce_dispatch(fx_ctx ctx, int kind, int upd, double a, fx_vec x, fx_vec y,
   if (x->dtype == FX_F64)
     hipLaunchKernelGGL(k_reduce<double>, dim3(g), dim3(kThreads), 0, ctx->stream, kind, upd, a,
                        (const double*)x->d, (double*)(y ? y->d : nullptr),
-                       (const double*)(z ? z->d : nullptr), x->n, ctx->ws.partials,
-                       ctx->ws.counter, dev_out);
+                       (const double*)(z ? z->d : nullptr), x->n, ctx->ws.partials);
   else
     hipLaunchKernelGGL(k_reduce<float>, dim3(g), dim3(kThreads), 0, ctx->stream, kind, upd, a,
                        (const float*)x->d, (float*)(y ? y->d : nullptr),
-                       (const float*)(z ? z->d : nullptr), x->n, ctx->ws.partials,
-                       ctx->ws.counter, dev_out);
+                       (const float*)(z ? z->d : nullptr), x->n, ctx->ws.partials);
   FX_HIP(hipGetLastError());
-  return 0;
+  if (kind == 2) return launch_finish<1, 1u>(ctx->stream, ctx->ws.partials, g, FinStore<1>{dev_out, 0});
+  return launch_finish<1, 0u>(ctx->stream, ctx->ws.partials, g, FinStore<1>{dev_out, kind == 1});
 }
 
 int fetch_scalar(fx_ctx ctx, int slot, double* out) {
@@ -204,8 +204,6 @@ void fx_params_default(fx_params* p) {
 
 static int ctx_init(fx_ctx c) {
   FX_HIP(hipMalloc(&c->ws.partials, sizeof(double) * 16 * 65536));
-  FX_HIP(hipMalloc(&c->ws.counter, sizeof(unsigned) * 64));
-  FX_HIP(hipMemset(c->ws.counter, 0, sizeof(unsigned) * 64));
   FX_HIP(hipMalloc(&c->dscal, sizeof(double) * 64));
   FX_HIP(hipHostMalloc(&c->hpin, sizeof(double) * 64, hipHostMallocDefault));
   hipDeviceProp_t prop;
@@ -256,7 +254,6 @@ int fx_ctx_destroy(fx_ctx c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->ws.partials);
-  (void)hipFree(c->ws.counter);
   (void)hipFree(c->dscal);
   if (c->hpin) (void)hipHostFree(c->hpin);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -72,13 +72,12 @@ __global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d
 // max |x_i| into *out (one reduction)
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_norminf(const T* __restrict__ x, uint32_t n,
-                                                      double* partials, unsigned* counter,
-                                                      double* out) {
+                                                      double* partials) {
   double acc[1] = {0.0};
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
     acc[0] = fmax(acc[0], fabs((double)x[i]));
-  if (grid_reduce<1, 1u>(acc, partials, counter) && threadIdx.x == 0) *out = acc[0];
+  block_partials<1, 1u>(acc, partials);
 }
 
 // r = b if ||x||_inf < CLOSE_ZERO, else the row chain b - A x
@@ -215,13 +214,11 @@ int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype) {
   if (A->nrow == 0) return 0;
   if (dtype == FX_F64)
     return launch_spmv<double, double, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<double>{(const double*)x},
-                                                      BNone{}, EpiStore<double>{(double*)y}, nullptr,
-                                                      nullptr);
+                                                      BNone{}, EpiStore<double>{(double*)y}, nullptr);
   FX_CHECK(A->va32 || !A->has_values, FX_ERROR_INPUT_PAR,
            "f32 SpMV needs the f32 value copy (fx_csr_make_f32)");
   return launch_spmv<float, float, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<float>{(const float*)x},
-                                                  BNone{}, EpiStore<float>{(float*)y}, nullptr,
-                                                  nullptr);
+                                                  BNone{}, EpiStore<float>{(float*)y}, nullptr);
 }
 
 }  // namespace fx
@@ -387,17 +384,19 @@ int fx_residual(fx_ctx ctx, fx_csr A, fx_vec b, fx_vec x, fx_vec r) {
   double* xinf = ctx->dscal + 1;
   if (x->dtype == FX_F64) {
     hipLaunchKernelGGL(k_norminf<double>, dim3(g), dim3(kThreads), 0, st, (const double*)x->d, x->n,
-                       ctx->ws.partials, ctx->ws.counter, xinf);
+                       ctx->ws.partials);
+    FX_TRY((launch_finish<1, 1u>(st, ctx->ws.partials, g, FinStore<1>{xinf, 0})));
     return launch_spmv<double, double, ROW_B_MINUS_AX, 0, 0u>(
         A, st, XLoad<double>{(const double*)x->d}, BLoad<double>{(const double*)b->d},
-        EpiResidual<double>{(double*)r->d, (const double*)b->d, xinf}, nullptr, nullptr);
+        EpiResidual<double>{(double*)r->d, (const double*)b->d, xinf}, nullptr);
   }
   FX_CHECK(A->va32 || !A->has_values, FX_ERROR_INPUT_PAR, "f32 residual needs fx_csr_make_f32");
   hipLaunchKernelGGL(k_norminf<float>, dim3(g), dim3(kThreads), 0, st, (const float*)x->d, x->n,
-                     ctx->ws.partials, ctx->ws.counter, xinf);
+                     ctx->ws.partials);
+  FX_TRY((launch_finish<1, 1u>(st, ctx->ws.partials, g, FinStore<1>{xinf, 0})));
   return launch_spmv<float, float, ROW_B_MINUS_AX, 0, 0u>(
       A, st, XLoad<float>{(const float*)x->d}, BLoad<float>{(const float*)b->d},
-      EpiResidual<float>{(float*)r->d, (const float*)b->d, xinf}, nullptr, nullptr);
+      EpiResidual<float>{(float*)r->d, (const float*)b->d, xinf}, nullptr);
 }
 
 int fx_csr_get_diag(fx_ctx ctx, fx_csr A, fx_vec v) {

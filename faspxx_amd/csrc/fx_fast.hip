@@ -9,13 +9,17 @@
 // safe-guard copy) stop the device loop and are run on the host with the
 // per-operation backend, after which the device loop resumes.
 //
-// CG iteration = 3 kernels, replayed as a hipGraph of kChunk iterations:
-//   K1  q = A p, (q, p)                 -> alpha            (SpMV + 1 reduction)
+// CG iteration = 5 kernels, replayed as a hipGraph of kChunk iterations:
+//   K1  q = A p, (q, p)                                  (SpMV + 1 reduction)
+//   F1  alpha = (z, r) / (q, p)                          (finisher, 1 workgroup)
 //   K2  x += alpha p, r -= alpha q, z = D r;  (r,r) (z,r) max|x| (x,x) (p,p)
-//                                       -> checks, beta     (stream + 5 reductions)
-//   K3  p = beta p + z                                       (stream)
-// Per iteration HBM traffic: SpMV + 13 vector passes (K1 2, K2 7, K3 4 in f64
-// units: p/q, x p r q D / x r, r D p / p).
+//                                                        (stream + 5 reductions)
+//   F2  the reference's end-of-iteration checks, beta     (finisher)
+//   K3  p = beta p + z                                    (stream)
+// Per iteration HBM traffic: the SpMV's matrix stream + 13 vector passes in
+// f64 units (K1: p read, q written; K2: x p r q D read, x r written; K3: r D
+// p read, p written).  Forming p inside K1 instead (saving K3) was measured
+// slower: it triples the gathered loads of the latency-bound SpMV.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -154,8 +158,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* 
                                                         const VT* __restrict__ p,
                                                         const VT* __restrict__ q,
                                                         const VT* __restrict__ dinv, uint32_t n,
-                                                        CgState* st, CgPrm P, double* hist,
-                                                        double* partials, unsigned* counter) {
+                                                        const CgState* st, double* partials) {
   if (st->status != ST_RUN) return;
   const VT a = (VT)st->alpha;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -173,9 +176,18 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* 
     acc[3] = fma((double)xi, (double)xi, acc[3]);
     acc[4] = fma((double)pi, (double)pi, acc[4]);
   }
-  if (grid_reduce<5, 4u>(acc, partials, counter) && threadIdx.x == 0)
-    cg_after_update(st, P, acc, hist);
+  block_partials<5, 4u>(acc, partials);
 }
+
+struct FinCgUpdate {
+  CgState* st;
+  CgPrm P;
+  double* hist;
+  __device__ bool skip() const { return st->status != ST_RUN; }
+  __device__ void fin(double (&s)[5]) const {
+    if (threadIdx.x == 0) cg_after_update(st, P, s, hist);
+  }
+};
 
 template <class VT, bool JAC>
 __global__ __launch_bounds__(kThreads) void k_cg_pupdate(VT* __restrict__ p,
@@ -195,8 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_pupdate(VT* __restrict__ p,
 template <class VT, bool JAC>
 __global__ __launch_bounds__(kThreads) void k_cg_init(VT* __restrict__ p, const VT* __restrict__ r,
                                                       const VT* __restrict__ dinv, uint32_t n,
-                                                      CgState* st, CgPrm P, double* hist,
-                                                      double* partials, unsigned* counter) {
+                                                      double* partials) {
   double acc[2] = {0.0, 0.0};
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
@@ -206,17 +217,26 @@ __global__ __launch_bounds__(kThreads) void k_cg_init(VT* __restrict__ p, const 
     acc[0] = fma((double)ri, (double)ri, acc[0]);
     acc[1] = fma((double)zi, (double)ri, acc[1]);
   }
-  if (grid_reduce<2>(acc, partials, counter) && threadIdx.x == 0) {
+  block_partials<2>(acc, partials);
+}
+
+struct FinCgInit {
+  CgState* st;
+  CgPrm P;
+  double* hist;
+  __device__ bool skip() const { return false; }
+  __device__ void fin(double (&s)[2]) const {
+    if (threadIdx.x != 0) return;
     CgState* S = st;
-    S->tmpa = acc[1];
-    S->rr = acc[0];
+    S->tmpa = s[1];
+    S->rr = s[0];
     if (P.maxIter == 0) {
       S->status = ST_DONE;
       return;
     }
-    cg_loop_top(S, P, acc[0], hist);
+    cg_loop_top(S, P, s[0], hist);
   }
-}
+};
 
 // ---- Jacobi solver kernels (src/Iter.cxx:69-99) ---------------------------
 
@@ -290,14 +310,21 @@ struct EpiResNorms {
 };
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_norminf_state(const T* __restrict__ x, uint32_t n,
-                                                            double* out, double* partials,
-                                                            unsigned* counter) {
+__global__ __launch_bounds__(kThreads) void k_norminf_part(const T* __restrict__ x, uint32_t n,
+                                                           double* partials) {
   double acc[1] = {0.0};
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
     acc[0] = fmax(acc[0], fabs((double)x[i]));
-  if (grid_reduce<1, 1u>(acc, partials, counter) && threadIdx.x == 0) *out = acc[0];
+  block_partials<1, 1u>(acc, partials);
+}
+
+// ||x||_inf into *out: partials kernel + finisher
+int norminf_to(hipStream_t st, const double* x, uint32_t n, double* out, double* partials) {
+  const unsigned g = stream_blocks(n);
+  hipLaunchKernelGGL(k_norminf_part<double>, dim3(g), dim3(kThreads), 0, st, x, n, partials);
+  FX_HIP(hipGetLastError());
+  return launch_finish<1, 1u>(st, partials, g, FinStore<1>{out, 0});
 }
 
 // ---- mixed precision helpers -----------------------------------------------
@@ -340,7 +367,6 @@ struct FastWs {
   void* q = nullptr;
   void* safe = nullptr;
   double* partials = nullptr;
-  unsigned* counters = nullptr;
   void* state = nullptr;  // CgState / JacState
   void* hstate = nullptr; // pinned, 2 slots
   double* hist = nullptr;
@@ -352,7 +378,7 @@ struct FastWs {
 
   ~FastWs() {
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
-    void* bufs[] = {x, x2, r, p, q, safe, partials, counters, state, hist, dscal};
+    void* bufs[] = {x, x2, r, p, q, safe, partials, state, hist, dscal};
     for (void* b : bufs) (void)hipFree(b);
     if (hstate) (void)hipHostFree(hstate);
     if (chunk) (void)hipGraphExecDestroy(chunk);
@@ -378,8 +404,6 @@ int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_
   FX_HIP(hipMemsetAsync(W.safe, 0, vb, ctx->stream));
   const size_t np = 8 * (size_t)((spmv_blocks > (uint32_t)kMaxRedBlocks) ? spmv_blocks : kMaxRedBlocks);
   FX_HIP(hipMalloc(&W.partials, sizeof(double) * np));
-  FX_HIP(hipMalloc(&W.counters, sizeof(unsigned) * 16));
-  FX_HIP(hipMemsetAsync(W.counters, 0, sizeof(unsigned) * 16, ctx->stream));
   FX_HIP(hipMalloc(&W.state, state_bytes));
   FX_HIP(hipHostMalloc(&W.hstate, 2 * state_bytes, hipHostMallocDefault));
   W.hist_cap = hist_cap + 2;
@@ -404,33 +428,32 @@ struct CgRun {
 
   CgState* dstate() const { return (CgState*)W.state; }
 
-  int launch_iteration() {
+  template <bool JAC>
+  int launch_iteration_t() {
     VT* p = (VT*)W.p;
     VT* q = (VT*)W.q;
     VT* x = (VT*)W.x;
     VT* r = (VT*)W.r;
     const uint32_t n = W.n;
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
-                                               EpiCgQ<VT>{q, p, dstate()}, W.partials,
-                                               W.counters + 0)));
+                                               EpiCgQ<VT>{q, p, dstate()}, W.partials)));
     const unsigned g = stream_blocks(n);
-    if (dinv) {
-      hipLaunchKernelGGL((k_cg_update<VT, true>), dim3(g), dim3(kThreads), 0, st, x, r, p, q, dinv,
-                         n, dstate(), P, W.hist, W.partials, W.counters + 1);
-      hipLaunchKernelGGL((k_cg_pupdate<VT, true>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
-                         dstate());
-    } else {
-      hipLaunchKernelGGL((k_cg_update<VT, false>), dim3(g), dim3(kThreads), 0, st, x, r, p, q,
-                         dinv, n, dstate(), P, W.hist, W.partials, W.counters + 1);
-      hipLaunchKernelGGL((k_cg_pupdate<VT, false>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
-                         dstate());
-    }
+    hipLaunchKernelGGL((k_cg_update<VT, JAC>), dim3(g), dim3(kThreads), 0, st, x, r, p, q, dinv, n,
+                       dstate(), W.partials);
+    FX_HIP(hipGetLastError());
+    FX_TRY((launch_finish<5, 4u>(st, W.partials, g, FinCgUpdate{dstate(), P, W.hist})));
+    hipLaunchKernelGGL((k_cg_pupdate<VT, JAC>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
+                       dstate());
     FX_HIP(hipGetLastError());
     return 0;
   }
+  int launch_iteration() { return dinv ? launch_iteration_t<true>() : launch_iteration_t<false>(); }
+
+  // The legacy null stream cannot be captured: replay by direct launches.
+  bool use_graph() const { return st != nullptr; }
 
   int build_graph() {
-    if (W.chunk) return 0;
+    if (W.chunk || !use_graph()) return 0;
     hipGraph_t g = nullptr;
     FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = 0;
@@ -445,7 +468,11 @@ struct CgRun {
   }
 
   int launch_chunk(int slot) {
-    FX_HIP(hipGraphLaunch(W.chunk, st));
+    if (use_graph()) {
+      FX_HIP(hipGraphLaunch(W.chunk, st));
+    } else {
+      for (int c = 0; c < kChunk; ++c) FX_TRY(launch_iteration());
+    }
     FX_HIP(hipMemcpyAsync((CgState*)W.hstate + slot, W.state, sizeof(CgState),
                           hipMemcpyDeviceToHost, st));
     FX_HIP(hipEventRecord(W.ev[slot], st));
@@ -580,15 +607,16 @@ struct CgRun {
     FX_TRY(write_state(s0));
     // r = b - A x;  z = P r;  p = z;  tmpa = (z, r);  loop top
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 0, 0u>(A, st, XLoad<VT>{(const VT*)W.x}, BNone{},
-                                               EpiBMinusW<VT>{(VT*)W.r, b}, nullptr, nullptr)));
+                                               EpiBMinusW<VT>{(VT*)W.r, b}, nullptr)));
     const unsigned g = stream_blocks(n);
     if (dinv)
       hipLaunchKernelGGL((k_cg_init<VT, true>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
-                         (const VT*)W.r, dinv, n, dstate(), P, W.hist, W.partials, W.counters + 2);
+                         (const VT*)W.r, dinv, n, W.partials);
     else
       hipLaunchKernelGGL((k_cg_init<VT, false>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
-                         (const VT*)W.r, dinv, n, dstate(), P, W.hist, W.partials, W.counters + 2);
+                         (const VT*)W.r, dinv, n, W.partials);
     FX_HIP(hipGetLastError());
+    FX_TRY((launch_finish<2, 0u>(st, W.partials, g, FinCgInit{dstate(), P, W.hist})));
     FX_TRY(build_graph());
 
     // Device-vector views for the host branches
@@ -603,6 +631,7 @@ struct CgRun {
     o.owned.push_back(vr);
     Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.p, &vp));
     o.owned.push_back(vp);
+
     Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.q, &vq));
     o.owned.push_back(vq);
     Ops::chk(fx_vec_wrap(W.ctx, n, dt, W.safe, &vs));
@@ -681,22 +710,24 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   s0.denAbs = 1.0;
   s0.status = P.maxIter == 0 ? ST_DONE : ST_RUN;
   FX_HIP(hipMemcpyAsync(ds, &s0, sizeof(JacState), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_norminf_state<double>, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
-                     (const double*)xa, n, &ds->xinf, W.partials, W.counters + 3);
-  FX_HIP(hipGetLastError());
+  FX_TRY(norminf_to(st, xa, n, &ds->xinf, W.partials));
   const double* dinv = S->pc->dinv;
-  if (!W.chunk) {
-    hipGraph_t g = nullptr;
-    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int c = 0; c < kChunk && !rc; ++c) {
+  auto sweeps = [&]() -> int {
+    for (int c = 0; c < kChunk; ++c) {
       const bool even = (c % 2) == 0;
       const double* src = even ? xa : xb;
       double* dst = even ? xb : xa;
-      rc = launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
+      FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
           A, st, XLoad<double>{src}, BLoad<double>{b},
-          EpiJacobi{dst, src, b, dinv, ds, P, even ? 0u : 1u, W.hist}, W.partials, W.counters + 4);
+          EpiJacobi{dst, src, b, dinv, ds, P, even ? 0u : 1u, W.hist}, W.partials)));
     }
+    return 0;
+  };
+  const bool graph = st != nullptr;  // the legacy null stream cannot be captured
+  if (graph && !W.chunk) {
+    hipGraph_t g = nullptr;
+    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = sweeps();
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc) return rc;
     FX_HIP(e);
@@ -704,11 +735,18 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
     (void)hipGraphDestroy(g);
     FX_HIP(e);
   }
+  auto replay = [&]() -> int {
+    if (graph) {
+      FX_HIP(hipGraphLaunch(W.chunk, st));
+      return 0;
+    }
+    return sweeps();
+  };
   // kChunk is even: every replay starts reading buffer A
   JacState s{};
   int cur = 0;
   for (int i = 0; i < 2; ++i) {
-    FX_HIP(hipGraphLaunch(W.chunk, st));
+    FX_TRY(replay());
     FX_HIP(hipMemcpyAsync((JacState*)W.hstate + i, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
     FX_HIP(hipEventRecord(W.ev[i], st));
   }
@@ -719,7 +757,7 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
     if (((const JacState*)W.hstate + cur)->status != ST_RUN) break;
     FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "Jacobi device loop did not stop");
     ++chunks;
-    FX_HIP(hipGraphLaunch(W.chunk, st));
+    FX_TRY(replay());
     FX_HIP(hipMemcpyAsync((JacState*)W.hstate + cur, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
     FX_HIP(hipEventRecord(W.ev[cur], st));
     cur ^= 1;
@@ -729,11 +767,10 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   double* ans = s.answer ? xb : xa;
   res->numIter = s.numIter;
   if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
-    hipLaunchKernelGGL(k_norminf_state<double>, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
-                       (const double*)ans, n, W.dscal, W.partials, W.counters + 5);
+    FX_TRY(norminf_to(st, ans, n, W.dscal, W.partials));
     FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
         A, st, XLoad<double>{ans}, BLoad<double>{b}, EpiResNorms{b, W.dscal, W.dscal + 1},
-        W.partials, W.counters + 6)));
+        W.partials)));
     double h[3];
     FX_HIP(hipMemcpyAsync(h, W.dscal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
     FX_HIP(hipStreamSynchronize(st));
@@ -764,7 +801,7 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
     S->fast = new FastWs();
     const size_t sb = S->type == FX_SOLVER_JACOBI ? sizeof(JacState) : sizeof(CgState);
     int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
-                      spmv_grid(S->A->nblk));
+                      spmv_grid(S->A->nblk, S->ctx->sms));
     if (rc) {
       delete S->fast;
       S->fast = nullptr;
@@ -800,7 +837,6 @@ struct fx_ir_s {
   float* r32 = nullptr;
   float* d32 = nullptr;
   double* dsc = nullptr;
-  unsigned* ctr = nullptr;
   double* part = nullptr;
   FastWs* W = nullptr;
   ~fx_ir_s() {
@@ -811,7 +847,6 @@ struct fx_ir_s {
     (void)hipFree(r32);
     (void)hipFree(d32);
     (void)hipFree(dsc);
-    (void)hipFree(ctr);
     (void)hipFree(part);
   }
 };
@@ -843,17 +878,16 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
   if (!rc) rc = fx_solver_setup(ir->inner_s, A, ir->pc, inner);
   if (rc) return fail(rc);
   const size_t nb = n ? n : 1;
-  const size_t np = 8 * ((size_t)spmv_grid(A->nblk) + kMaxRedBlocks);
+  const size_t np = 8 * ((size_t)spmv_grid(A->nblk, ctx->sms) + kMaxRedBlocks);
   if (hipMalloc(&ir->r32, sizeof(float) * nb) || hipMalloc(&ir->d32, sizeof(float) * nb) ||
-      hipMalloc(&ir->dsc, sizeof(double) * 4) || hipMalloc(&ir->ctr, sizeof(unsigned) * 4) ||
+      hipMalloc(&ir->dsc, sizeof(double) * 4) ||
       hipMalloc(&ir->part, sizeof(double) * np)) {
     set_error("IR workspace allocation failed");
     return fail(FX_ERROR_ALLOC_MEM);
   }
-  if (hipMemsetAsync(ir->ctr, 0, sizeof(unsigned) * 4, ctx->stream)) return fail(FX_ERROR_DEVICE);
   if (inner_type == FX_SOLVER_CG) {
     ir->W = new FastWs();
-    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sizeof(CgState), spmv_grid(A->nblk));
+    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sizeof(CgState), spmv_grid(A->nblk, ctx->sms));
     if (rc) return fail(rc);
   }
   *out = ir;
@@ -882,7 +916,7 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
       // r = b - A x (f64 values, Apply + XPAY order), r32 = (float) r, ||r||
       rc = launch_spmv<double, double, ROW_AX, 1, 0u>(
           A, st, XLoad<double>{(const double*)x->d}, BNone{},
-          EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part, ir->ctr);
+          EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part);
       if (rc) break;
       double h = 0;
       if (hipMemcpyAsync(&h, ir->dsc, sizeof(double), hipMemcpyDeviceToHost, st) ||

@@ -10,6 +10,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "faspxx_hip.h"
@@ -58,13 +60,10 @@ void clear_error();
 // Handles
 // ---------------------------------------------------------------------------
 
-// Reduction workspace: partial sums of one reduction kernel are written to
-// `partials` ([slot][block]); the last workgroup to finish (ticket on
-// `counter`) reduces them in a fixed order and resets the counter, so the
-// result is deterministic and graph replays need no memset.
+// Reduction workspace: per-workgroup partials of a reduction kernel
+// ([slot][workgroup]), combined by a one-workgroup finisher kernel.
 struct fx_red_ws {
-  double* partials = nullptr;  // kMaxRedSlots * kMaxRedBlocks
-  unsigned* counter = nullptr;
+  double* partials = nullptr;  // 16 slots x 65536 workgroups
 };
 
 struct fx_ctx_s {
@@ -115,9 +114,12 @@ struct fx_pc_s {
 
 namespace fx {
 
-// Row-block geometry of the SpMV kernels.
+// Row-block geometry of the SpMV kernels (LDS stage of kBlockNnz entries).
+#ifndef FX_BLOCK_NNZ
+#define FX_BLOCK_NNZ 2048
+#endif
 constexpr uint32_t kBlockRows = kThreads;
-constexpr uint32_t kBlockNnz = 2048;
+constexpr uint32_t kBlockNnz = FX_BLOCK_NNZ;
 
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype);
 int ensure_dinv32(fx_pc pc);
@@ -130,9 +132,10 @@ inline unsigned stream_blocks(uint32_t n, uint32_t per_thread = 4) {
   return (unsigned)b;
 }
 
-// Grid for the row-block SpMV kernels: padded to a multiple of the XCD count
-// so that xcd_block() can give each XCD one contiguous range of row blocks.
-inline unsigned spmv_grid(uint32_t nblk) {
+// Grid of the row-block SpMV kernels: one workgroup per row block, padded to
+// a multiple of the XCD count so that xcd_block() can give each XCD one
+// contiguous range of row blocks.
+inline unsigned spmv_grid(uint32_t nblk, int /*cus*/) {
   unsigned g = nblk ? nblk : 1;
   return (g + kXcds - 1) / kXcds * kXcds;
 }
@@ -144,8 +147,12 @@ inline unsigned spmv_grid(uint32_t nblk) {
 // Workgroup id -> logical row block.  Hardware hands workgroup i to XCD
 // i % 8; this maps XCD k to the contiguous logical range [k*G/8, (k+1)*G/8),
 // so neighbouring row blocks (which share x entries) share an L2.
+#ifndef FX_XCD_REMAP
+#define FX_XCD_REMAP 1
+#endif
 __device__ __forceinline__ unsigned xcd_block() {
   const unsigned g = gridDim.x, i = blockIdx.x;
+  if (!FX_XCD_REMAP) return i;
   return (i % kXcds) * (g / kXcds) + i / kXcds;
 }
 
@@ -190,40 +197,97 @@ __device__ __forceinline__ void block_reduce(double (&v)[K]) {
   __syncthreads();
 }
 
-// Two-stage grid reduction.  Every workgroup of the grid calls this once with
-// its per-thread values.  Returns true in all threads of the last workgroup
-// to arrive, with v holding the grid-wide results; false elsewhere.  The
-// partials of workgroup b for slot k live at partials[k * gridDim.x + b].
+// Two-stage grid reduction without fences or atomics.  Stage 1: every
+// workgroup of a producer kernel reduces its per-thread values and writes
+// one partial per slot to partials[k * gridDim.x + blockIdx.x].  Stage 2: a
+// one-workgroup finisher kernel (k_finish) launched right after it on the
+// same stream reduces the G partials in a fixed order and runs the caller's
+// finisher.  The kernel boundary makes the partials visible; the order of
+// summation depends only on G, so results are deterministic.  (A per-workgroup
+// device-scope fence would write back the whole L2 on gfx950, for every
+// workgroup.)
 template <int K, unsigned MaxMask = 0u>
-__device__ __forceinline__ bool grid_reduce(double (&v)[K], double* partials, unsigned* counter) {
-  __shared__ int is_last;
+__device__ __forceinline__ void block_partials(double (&v)[K], double* partials) {
   block_reduce<K, MaxMask>(v);
-  const unsigned G = gridDim.x;
   if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[k * G + blockIdx.x] = v[k];
-    __threadfence();
-    const unsigned ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == G - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Finisher functor: bool skip() const; void fin(double (&sum)[K]) const (all
+// threads of the finisher workgroup call it; it usually acts in thread 0).
+// 1024 threads, each with kFinUnroll independent loads in flight, so the
+// partials (up to ~16K per slot) are read in a few memory latencies.
+constexpr int kFinThreads = 1024;
+constexpr int kFinUnroll = 8;
+template <int K, unsigned MaxMask, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict__ partials,
+                                                        unsigned G, Fin f) {
+  if (f.skip()) return;
+  __shared__ double red[K][kFinThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    const bool mx = (MaxMask >> k) & 1u;
     double a = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += kThreads) {
-      const double p = partials[k * G + b];
-      a = ((MaxMask >> k) & 1u) ? fmax(a, p) : a + p;
+    for (unsigned base = 0; base < G; base += kFinThreads * kFinUnroll) {
+      double v[kFinUnroll];
+#pragma unroll
+      for (int j = 0; j < kFinUnroll; ++j) {
+        const unsigned b = base + tid + j * kFinThreads;
+        v[j] = b < G ? partials[k * G + b] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < kFinUnroll; ++j) a = mx ? fmax(a, v[j]) : a + v[j];
     }
+    a = mx ? wave_max(a) : wave_sum(a);
+    if (lane == 0) red[k][wid] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool mx = (MaxMask >> k) & 1u;
+    double a = red[k][0];
+#pragma unroll
+    for (int w = 1; w < kFinThreads / 64; ++w) a = mx ? fmax(a, red[k][w]) : a + red[k][w];
     acc[k] = a;
   }
-  block_reduce<K, MaxMask>(acc);
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = acc[k];
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
+  f.fin(acc);
 }
+
+template <int K, unsigned MaxMask, class Fin>
+inline int launch_finish(hipStream_t st, const double* partials, unsigned G, Fin f) {
+  hipLaunchKernelGGL((k_finish<K, MaxMask, Fin>), dim3(1), dim3(kFinThreads), 0, st, partials, G,
+                     f);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Optional per-thread set-up of kernel functors (e.g. loading a device scalar
+// once): called when the functor has an init() member.
+template <class T, class = void>
+struct has_init : std::false_type {};
+template <class T>
+struct has_init<T, std::void_t<decltype(std::declval<T&>().init())>> : std::true_type {};
+template <class T>
+__device__ __forceinline__ void maybe_init(T& t) {
+  if constexpr (has_init<T>::value) t.init();
+}
+
+// Finisher that stores the reduced slots (optionally sqrt of slot 0).
+template <int K>
+struct FinStore {
+  double* out;
+  int sqrt0;
+  __device__ bool skip() const { return false; }
+  __device__ void fin(double (&s)[K]) const {
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = (k == 0 && sqrt0) ? sqrt(s[k]) : s[k];
+  }
+};
 
 }  // namespace fx
