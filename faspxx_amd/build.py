@@ -68,7 +68,31 @@ def build(verbose: bool = False) -> pathlib.Path:
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stdout}\n{p.stderr}")
     os.replace(tmp, OUT)
+    build_dropin()
     return OUT
+
+
+DROPIN = PKG / "libfaspxx_dropin.so"
+DROPIN_TEST = ROOT / "tests" / "cpp" / "dropin_tests"
+
+
+def build_dropin() -> None:
+    """The FASP++-interface C++ classes (host code over the C ABI) and the
+    C++ port of the reference's unit tests that exercises them."""
+    cxx = os.environ.get("CXX", "g++")
+    common = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}",
+              f"-I{PKG / 'host'}"]
+    cmd = [cxx, *common, "-shared", "-o", str(DROPIN), str(PKG / "host" / "faspxx_dropin.cpp"),
+           f"-L{PKG}", "-lfaspxx_hip", "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"drop-in library build failed:\n{p.stderr}")
+    cmd = [cxx, *common, "-o", str(DROPIN_TEST), str(ROOT / "tests" / "cpp" / "dropin_tests.cpp"),
+           f"-L{PKG}", "-lfaspxx_dropin", "-lfaspxx_hip",
+           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../faspxx_amd"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{p.stderr}")
 
 
 if __name__ == "__main__":

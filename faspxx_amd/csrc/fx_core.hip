@@ -23,7 +23,10 @@ void clear_error() { g_err[0] = 0; }
 // Elementwise kernels.  Each thread handles elements i = tid, tid + stride, ...
 // Expressions follow the reference's contracted forms (src/VEC.cxx).
 // ---------------------------------------------------------------------------
-enum ElemOp { OP_AXPY, OP_XPAY, OP_AXPBY, OP_WAXPBY, OP_SCALE, OP_PMUL, OP_RECIP, OP_ADD, OP_SET };
+enum ElemOp {
+  OP_AXPY, OP_XPAY, OP_AXPBY, OP_WAXPBY, OP_SCALE, OP_PMUL, OP_RECIP, OP_ADD, OP_SET,
+  OP_SHIFT, OP_ABS, OP_PDIV, OP_SUB
+};
 
 template <class T>
 __device__ __forceinline__ T elem(int op, int form, double a, double b, T x, T y) {
@@ -44,6 +47,10 @@ __device__ __forceinline__ T elem(int op, int form, double a, double b, T x, T y
     case OP_RECIP: return (T)1 / y;
     case OP_ADD: return y + x;
     case OP_SET: return (T)a;
+    case OP_SHIFT: return y + (T)a;  // VEC::Shift, src/VEC.cxx:217-228
+    case OP_ABS: return fabs(y);     // VEC::Abs, src/VEC.cxx:231-242
+    case OP_PDIV: return y / x;      // VEC::PointwiseDivide, src/VEC.cxx:196-207
+    case OP_SUB: return y - x;       // VEC::operator-=, src/VEC.cxx:81-93
     default: return y;
   }
 }
@@ -85,7 +92,9 @@ __global__ __launch_bounds__(kThreads) void k_convert(const S* __restrict__ s, D
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) d[i] = (D)s[i];
 }
 
-// Reductions: kind 0 dot(x,y), 1 sum x^2 (then sqrt), 2 max |x|.  Optional
+// Reductions: kind 0 dot(x,y), 1 sum x^2 (then sqrt), 2 max |x|, 3 max x
+// (from SMALL_DBL = -1e60 as VEC::Max does, src/VEC.cxx:382-400), 4 min x
+// (from LARGE_DBL = 1e60, VEC::Min, :403-421; computed as -max(-x)).  Optional
 // fused update y += a x before the reduction (kinds 0/1 with `upd`).
 // Products are formed in double; sums accumulate in double with fma.  Each
 // workgroup writes one partial; k_finish combines them.
@@ -94,11 +103,15 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int kind, int upd, double a
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      const T* __restrict__ z, uint32_t n,
                                                      double* partials) {
-  double acc[1] = {0.0};
+  double acc[1] = {kind >= 3 ? -1e60 : 0.0};
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
     if (kind == 2) {
       acc[0] = fmax(acc[0], fabs((double)x[i]));
+    } else if (kind == 3) {
+      acc[0] = fmax(acc[0], (double)x[i]);
+    } else if (kind == 4) {
+      acc[0] = fmax(acc[0], -(double)x[i]);
     } else if (upd) {
       const T yn = fma((T)a, x[i], y[i]);
       y[i] = yn;
@@ -109,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int kind, int upd, double a
       acc[0] = fma(u, kind == 0 ? (double)y[i] : u, acc[0]);
     }
   }
-  if (kind == 2)
+  if (kind >= 2)
     block_partials<1, 1u>(acc, partials);
   else
     block_partials<1, 0u>(acc, partials);
@@ -151,7 +164,9 @@ int reduce_dispatch(fx_ctx ctx, int kind, int upd, double a, fx_vec x, fx_vec y,
                        (const float*)x->d, (float*)(y ? y->d : nullptr),
                        (const float*)(z ? z->d : nullptr), x->n, ctx->ws.partials);
   FX_HIP(hipGetLastError());
-  if (kind == 2) return launch_finish<1, 1u>(ctx->stream, ctx->ws.partials, g, FinStore<1>{dev_out, 0});
+  if (kind >= 2)
+    return launch_finish<1, 1u>(ctx->stream, ctx->ws.partials, g,
+                                FinStore<1>{dev_out, kind == 4 ? 2 : 0});
   return launch_finish<1, 0u>(ctx->stream, ctx->ws.partials, g, FinStore<1>{dev_out, kind == 1});
 }
 
@@ -425,6 +440,33 @@ int fx_norminf(fx_ctx ctx, fx_vec x, double* out) {
   FX_TRY(reduce_dispatch(ctx, 2, 0, 0.0, x, nullptr, nullptr, ctx->dscal));
   return fetch_scalar(ctx, 0, out);
 }
+int fx_max(fx_ctx ctx, fx_vec x, double* out) {
+  clear_error();
+  FX_TRY(reduce_dispatch(ctx, 3, 0, 0.0, x, nullptr, nullptr, ctx->dscal));
+  return fetch_scalar(ctx, 0, out);
+}
+int fx_min(fx_ctx ctx, fx_vec x, double* out) {
+  clear_error();
+  FX_TRY(reduce_dispatch(ctx, 4, 0, 0.0, x, nullptr, nullptr, ctx->dscal));
+  return fetch_scalar(ctx, 0, out);
+}
+int fx_shift(fx_ctx ctx, double a, fx_vec y) {
+  clear_error();
+  return elem_dispatch(ctx, OP_SHIFT, 0, a, 0.0, nullptr, y);
+}
+int fx_abs(fx_ctx ctx, fx_vec y) {
+  clear_error();
+  return elem_dispatch(ctx, OP_ABS, 0, 0.0, 0.0, nullptr, y);
+}
+int fx_pdiv(fx_ctx ctx, fx_vec v, fx_vec y) {
+  clear_error();
+  return elem_dispatch(ctx, OP_PDIV, 0, 0.0, 0.0, v, y);
+}
+int fx_sub(fx_ctx ctx, fx_vec v, fx_vec y) {
+  clear_error();
+  return elem_dispatch(ctx, OP_SUB, 0, 0.0, 0.0, v, y);
+}
+
 int fx_axpy_norm2(fx_ctx ctx, double a, fx_vec x, fx_vec y, double* out) {
   clear_error();
   FX_CHECK(y, FX_ERROR_INPUT_PAR, "null vector");

@@ -232,13 +232,14 @@ __global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool mx = (MaxMask >> k) & 1u;
-    double a = 0.0;
+    const double ident = mx ? -INFINITY : 0.0;
+    double a = ident;
     for (unsigned base = 0; base < G; base += kFinThreads * kFinUnroll) {
       double v[kFinUnroll];
 #pragma unroll
       for (int j = 0; j < kFinUnroll; ++j) {
         const unsigned b = base + tid + j * kFinThreads;
-        v[j] = b < G ? partials[k * G + b] : 0.0;
+        v[j] = b < G ? partials[k * G + b] : ident;
       }
 #pragma unroll
       for (int j = 0; j < kFinUnroll; ++j) a = mx ? fmax(a, v[j]) : a + v[j];
@@ -277,16 +278,18 @@ __device__ __forceinline__ void maybe_init(T& t) {
   if constexpr (has_init<T>::value) t.init();
 }
 
-// Finisher that stores the reduced slots (optionally sqrt of slot 0).
+// Finisher that stores the reduced slots; slot 0 optionally transformed
+// (post0 = 1: sqrt, 2: negate).
 template <int K>
 struct FinStore {
   double* out;
-  int sqrt0;
+  int post0;
   __device__ bool skip() const { return false; }
   __device__ void fin(double (&s)[K]) const {
     if (threadIdx.x != 0) return;
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = (k == 0 && sqrt0) ? sqrt(s[k]) : s[k];
+    for (int k = 0; k < K; ++k)
+      out[k] = (k == 0 && post0 == 1) ? sqrt(s[k]) : (k == 0 && post0 == 2) ? -s[k] : s[k];
   }
 };
 

@@ -724,10 +724,13 @@ int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm) {
   FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "solvers need a square matrix");
   if (s->type == FX_SOLVER_JACOBI)
     FX_CHECK(pc->kind == 1, FX_ERROR_SOLVER_PCD_TYPE, "Jacobi solver needs a Jacobi pc");
+  const fx_params np = prm ? *prm : s->prm;
+  // keep the fused-path workspace (and its replay graph) when nothing changed
+  const bool same = s->A == A && s->pc == pc && std::memcmp(&np, &s->prm, sizeof(fx_params)) == 0;
   s->A = A;
   s->pc = pc;
-  if (prm) s->prm = *prm;
-  s->release_fast();
+  s->prm = np;
+  if (!same) s->release_fast();
   return 0;
 }
 

@@ -94,6 +94,12 @@ _SIGS = {
     "fx_pmul": (C.c_int, [_P, _P, _P]),
     "fx_recip": (C.c_int, [_P, _P]),
     "fx_add": (C.c_int, [_P, _P, _P]),
+    "fx_shift": (C.c_int, [_P, C.c_double, _P]),
+    "fx_abs": (C.c_int, [_P, _P]),
+    "fx_pdiv": (C.c_int, [_P, _P, _P]),
+    "fx_sub": (C.c_int, [_P, _P, _P]),
+    "fx_max": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fx_min": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fx_dot": (C.c_int, [_P, _P, _P, C.POINTER(C.c_double)]),
     "fx_norm2": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fx_norminf": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
@@ -284,6 +290,29 @@ class Vec:
     def __iadd__(self, v: "Vec"):
         _chk(load().fx_add(self.ctx.h, v.h, self.h))
         return self
+
+    def Shift(self, a: float):
+        _chk(load().fx_shift(self.ctx.h, a, self.h))
+
+    def Abs(self):
+        _chk(load().fx_abs(self.ctx.h, self.h))
+
+    def PointwiseDivide(self, v: "Vec"):
+        _chk(load().fx_pdiv(self.ctx.h, v.h, self.h))
+
+    def __isub__(self, v: "Vec"):
+        _chk(load().fx_sub(self.ctx.h, v.h, self.h))
+        return self
+
+    def Max(self) -> float:
+        out = C.c_double()
+        _chk(load().fx_max(self.ctx.h, self.h, C.byref(out)))
+        return out.value
+
+    def Min(self) -> float:
+        out = C.c_double()
+        _chk(load().fx_min(self.ctx.h, self.h, C.byref(out)))
+        return out.value
 
     def Dot(self, v: "Vec") -> float:
         out = C.c_double()

@@ -1,0 +1,3 @@
+// MAT.hxx -- the reference header name, forwarding to the drop-in classes.
+#pragma once
+#include "faspxx_dropin.hxx"

@@ -133,6 +133,17 @@ int fx_pmul(fx_ctx ctx, fx_vec v, fx_vec y);
 int fx_recip(fx_ctx ctx, fx_vec y);
 /* y += v            VEC::operator+=, src/VEC.cxx:66-78 */
 int fx_add(fx_ctx ctx, fx_vec v, fx_vec y);
+/* y += a           VEC::Shift, src/VEC.cxx:217-228 */
+int fx_shift(fx_ctx ctx, double a, fx_vec y);
+/* y = |y|           VEC::Abs, src/VEC.cxx:231-242 */
+int fx_abs(fx_ctx ctx, fx_vec y);
+/* y ./= v           VEC::PointwiseDivide, src/VEC.cxx:196-207 */
+int fx_pdiv(fx_ctx ctx, fx_vec v, fx_vec y);
+/* y -= v            VEC::operator-=, src/VEC.cxx:81-93 */
+int fx_sub(fx_ctx ctx, fx_vec v, fx_vec y);
+/* max / min entry   VEC::Max / Min, src/VEC.cxx:382-421 (start values -1e60 / 1e60) */
+int fx_max(fx_ctx ctx, fx_vec x, double* out);
+int fx_min(fx_ctx ctx, fx_vec x, double* out);
 /* Reductions returning a host double (synchronising), VEC::Dot / Norm2 /
  * NormInf, src/VEC.cxx:424-485.  Summation order is a fixed two-stage tree
  * (deterministic run to run); it differs from the reference's 4-way order. */

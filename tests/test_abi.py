@@ -43,3 +43,14 @@ def test_params_default_matches_solparams():
     assert (p.maxIter, p.minIter, p.savIter, p.restart) == (1, 0, 100000, 30)
     assert (p.relTol, p.absTol) == (1e-6, 1e-8)
     assert (p.maxRestart, p.minRestart, p.leftPrecond) == (30, 10, 0)
+
+
+def test_dropin_library_builds_and_links():
+    """The C++ drop-in layer is built and links against the C ABI library."""
+    import ctypes
+    lib = ROOT / "faspxx_amd" / "libfaspxx_dropin.so"
+    if not lib.exists():
+        pytest.fail(f"{lib} missing: run __graft_entry__.build()")
+    ctypes.CDLL(str(ROOT / "faspxx_amd" / "libfaspxx_hip.so"))
+    ctypes.CDLL(str(lib))
+    assert (ROOT / "tests" / "cpp" / "dropin_tests").exists()
