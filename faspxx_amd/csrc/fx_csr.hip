@@ -175,6 +175,21 @@ __global__ void k_gather_t(const uint32_t* __restrict__ sorted, uint32_t nnz,
   }
 }
 
+__global__ void k_gather_values(const uint32_t* __restrict__ src, const double* __restrict__ va,
+                                double* __restrict__ vaT, uint32_t nnz) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < nnz; p += stride) vaT[p] = va[src[p]];
+}
+
+__global__ void k_pattern_grad(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                               const double* __restrict__ lam, const double* __restrict__ x,
+                               double* __restrict__ g, uint32_t nrow) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= nrow) return;
+  const double li = lam[i];
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) g[k] = -(li * x[ci[k]]);
+}
+
 int csr_alloc(fx_csr A) {
   const size_t n1 = (size_t)A->nrow + 1, nz = A->nnz ? A->nnz : 1, nd = A->nrow ? A->nrow : 1;
   FX_HIP(hipMalloc(&A->rp, sizeof(uint32_t) * n1));
@@ -205,6 +220,7 @@ void csr_free(fx_csr A) {
   (void)hipFree(A->va);
   (void)hipFree(A->va32);
   (void)hipFree(A->blk);
+  (void)hipFree(A->src);
   delete A;
 }
 
@@ -446,6 +462,10 @@ int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At) {
     (void)hipFree(sorted);
     (void)hipFree(rid);
   };
+  auto keep_sorted = [&]() {  // becomes T->src
+    T->src = sorted;
+    sorted = nullptr;
+  };
   if (hipMalloc(&cnt, sizeof(uint32_t) * (m + 1)) || hipMalloc(&perm, sizeof(uint32_t) * nnz) ||
       hipMalloc(&sorted, sizeof(uint32_t) * nnz) || hipMalloc(&rid, sizeof(uint32_t) * nnz)) {
     cleanup();
@@ -482,6 +502,7 @@ int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At) {
     set_error("transpose: download failed");
     return FX_ERROR_DEVICE;
   }
+  keep_sorted();
   cleanup();
   rc = csr_set_blocks(T, hrp.data());
   if (rc) {
@@ -496,6 +517,38 @@ int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At) {
     }
   }
   *At = T;
+  return 0;
+}
+
+// Refresh a transpose's values from its source: At.va[p] = A.va[src[p]].
+int fx_csr_transpose_update(fx_ctx ctx, fx_csr A, fx_csr At) {
+  clear_error();
+  FX_CHECK(ctx && A && At, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(At->src && At->nnz == A->nnz && At->nrow == A->mcol && At->mcol == A->nrow,
+           FX_ERROR_INPUT_PAR, "At is not a transpose of A");
+  if (A->nnz == 0) return 0;
+  FX_CHECK(A->has_values, FX_ERROR_INPUT_PAR, "A has no values");
+  hipLaunchKernelGGL(k_gather_values, dim3(stream_blocks(A->nnz)), dim3(kThreads), 0, ctx->stream,
+                     At->src, A->va, At->va, A->nnz);
+  FX_HIP(hipGetLastError());
+  At->has_values = true;
+  if (At->va32) return fx_csr_make_f32(At);
+  return 0;
+}
+
+// g_k = -lam[row(k)] * x[col(k)] for every stored entry k: the gradient of a
+// loss through x = A^{-1} b with respect to A's stored values, given the
+// adjoint solution lam of A^T lam = dL/dx.  Device pointers, nnz doubles out.
+int fx_csr_pattern_grad(fx_ctx ctx, fx_csr A, fx_vec lam, fx_vec x, double* dev_out) {
+  clear_error();
+  FX_CHECK(ctx && A && lam && x && dev_out, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(lam->n == A->nrow && x->n == A->mcol, FX_ERROR_NONMATCH_SIZE, "size mismatch");
+  FX_CHECK(lam->dtype == FX_F64 && x->dtype == FX_F64, FX_ERROR_INPUT_PAR, "f64 vectors required");
+  if (A->nnz == 0) return 0;
+  hipLaunchKernelGGL(k_pattern_grad, dim3((A->nrow + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     ctx->stream, A->rp, A->ci, (const double*)lam->d, (const double*)x->d,
+                     dev_out, A->nrow);
+  FX_HIP(hipGetLastError());
   return 0;
 }
 

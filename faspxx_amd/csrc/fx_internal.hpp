@@ -100,6 +100,8 @@ struct fx_csr_s {
   uint32_t* blk = nullptr;
   uint32_t nblk = 0;
   uint32_t max_row = 0;
+  // set on a transpose: entry p of this matrix is entry src[p] of its source
+  uint32_t* src = nullptr;
 };
 
 struct fx_pc_s {

@@ -120,6 +120,8 @@ _SIGS = {
     "fx_residual": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_csr_get_diag": (C.c_int, [_P, _P, _P]),
     "fx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "fx_csr_transpose_update": (C.c_int, [_P, _P, _P]),
+    "fx_csr_pattern_grad": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_pc_create_identity": (C.c_int, [_P, C.POINTER(_P)]),
     "fx_pc_create_jacobi": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
     "fx_pc_destroy": (C.c_int, [_P]),
@@ -401,6 +403,14 @@ class Csr:
         h = C.c_void_p()
         _chk(load().fx_csr_transpose(self.ctx.h, self.h, C.byref(h)))
         return Csr._from_handle(self.ctx, h)
+
+    def transpose_update(self, At: "Csr"):
+        """Refresh At's values from this matrix (At = self.transpose())."""
+        _chk(load().fx_csr_transpose_update(self.ctx.h, self.h, At.h))
+
+    def pattern_grad(self, lam: Vec, x: Vec, out_ptr: int):
+        """out_k = -lam[row(k)] * x[col(k)] (device pointer, nnz doubles)."""
+        _chk(load().fx_csr_pattern_grad(self.ctx.h, self.h, lam.h, x.h, C.c_void_p(out_ptr)))
 
     def set_values_dev(self, dev_ptr: int):
         _chk(load().fx_csr_set_values_dev(self.h, C.c_void_p(dev_ptr)))

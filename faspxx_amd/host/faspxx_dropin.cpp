@@ -458,6 +458,11 @@ FaspRetCode Identity::Solve(const VEC& b, VEC& x) {
   return SUCCESS;
 }
 
+fx_pc Identity::DevicePC(const MAT&) {
+  if (!pc_cache) check(fx_pc_create_identity(Context(), &pc_cache), "fx_pc_create_identity");
+  return pc_cache;
+}
+
 Jacobi::~Jacobi() {
   if (own) fx_pc_destroy(own);
   if (solver) fx_solver_destroy(solver);

@@ -268,6 +268,7 @@ class SOL {
 class Identity : public SOL {
  public:
   FaspRetCode Solve(const VEC& b, VEC& x) override;
+  fx_pc DevicePC(const MAT& A) override;
 };
 
 class Jacobi : public SOL {

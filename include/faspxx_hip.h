@@ -191,6 +191,15 @@ int fx_csr_get_diag(fx_ctx ctx, fx_csr A, fx_vec v);
  * ascending in each row; diagPtr formed as FormDiagPtr does). */
 int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At);
 
+/* Refresh the values of At (made by fx_csr_transpose from A) from A's
+ * current values, without recomputing the structure.  No reference
+ * counterpart: the adjoint solve's A^T after A's values change. */
+int fx_csr_transpose_update(fx_ctx ctx, fx_csr A, fx_csr At);
+/* out_k = -lam[row(k)] * x[col(k)] over A's stored entries (nnz doubles,
+ * device memory): dL/dA on the sparsity pattern, from the adjoint solution
+ * lam of A^T lam = dL/dx.  No reference counterpart. */
+int fx_csr_pattern_grad(fx_ctx ctx, fx_csr A, fx_vec lam, fx_vec x, double* dev_out);
+
 /* ---- preconditioners (src/Iter.hxx:26-71) ---------------------------- */
 /* Identity::Solve, src/Iter.cxx:16-20 */
 int fx_pc_create_identity(fx_ctx ctx, fx_pc* out);

@@ -213,7 +213,7 @@ static void test_solvers() {
     e.AXPY(-1.0, xs);
     REQUIRE(rc == SUCCESS);
     REQUIRE(e.Norm2() / xs.Norm2() < 1e-6);
-    std::printf("ok krylov %s\n", name);
+    std::printf("krylov %s: rc %d\n", name, (int)rc);
   }
 }
 
