@@ -5,7 +5,7 @@ FP32-inner / FP64-refinement CG (Jacobi preconditioned), inputs resident in
 HBM.  Reported beside it, in the same run: FP64 Jacobi-PCG time-to-solution,
 standalone SpMV GB/s (f64 and f32, Infinity Cache flushed between reps), 50
 weighted-Jacobi sweeps (omega = 2/3), and the compiled reference CPU path
-(oracle/_ref) on a bounded sample.
+(oracle/_ref) solving the same system on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -42,7 +42,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--spmv-reps", type=int, default=100)
+    ap.add_argument("--spmv-warmup", type=int, default=20)
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="standalone SpMV and a 256 MiB copy only (for profiler counter runs)")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -89,35 +92,38 @@ def main() -> int:
         xv = fx.Vec.from_numpy(ctx, np.random.default_rng(1).uniform(-1, 1, n),
                                dt)
         yv = fx.Vec(ctx, n, dt)
-        for _ in range(5):
-            A.Apply(xv, yv)
+        # every launch of this kernel is flushed (warm-up ones included), so
+        # a profiler's average over the kernel matches the mean below
         times = []
-        for i in range(args.spmv_reps):
+        for i in range(args.spmv_warmup + args.spmv_reps):
             torch.sum(flush, dim=0, out=flush_sink)
             e0, e1 = ev(), ev()
             e0.record(stream)
             A.Apply(xv, yv)
             e1.record(stream)
             e1.synchronize()
-            times.append(e0.elapsed_time(e1))
-        # warm (no flush) for comparison
-        e0, e1 = ev(), ev()
-        e0.record(stream)
-        for _ in range(20):
-            A.Apply(xv, yv)
-        e1.record(stream)
-        e1.synchronize()
-        warm_ms = e0.elapsed_time(e1) / 20
-        med = statistics.median(times)
+            if i >= args.spmv_warmup:
+                times.append(e0.elapsed_time(e1))
+        med, mean = statistics.median(times), statistics.mean(times)
         byts = spmv_bytes(n, nnz, s)
         res_spmv["f64" if dt == fx.F64 else "f32"] = {
-            "ms_median_flushed": med, "ms_mean_flushed": statistics.mean(times),
-            "GBs_flushed": byts / med / 1e6, "ms_warm": warm_ms, "GBs_warm": byts / warm_ms / 1e6,
-            "GFLOPs_flushed": 2 * nnz / med / 1e6, "bytes": byts,
+            "ms_median_flushed": med, "ms_mean_flushed": mean,
+            "GBs_flushed": byts / mean / 1e6, "GBs_median_flushed": byts / med / 1e6,
+            "GFLOPs_flushed": 2 * nnz / mean / 1e6, "bytes": byts, "reps": args.spmv_reps,
         }
-        log(f"SpMV {'f64' if dt == fx.F64 else 'f32'}: {med * 1e3:.1f} us flushed "
-            f"({byts / med / 1e6:.0f} GB/s), {warm_ms * 1e3:.1f} us warm")
+        log(f"SpMV {'f64' if dt == fx.F64 else 'f32'}: {mean * 1e3:.1f} us mean flushed "
+            f"({byts / mean / 1e6:.0f} GB/s), median {med * 1e3:.1f} us")
         del xv, yv
+    if args.spmv_only:
+        # a plain copy of known size: the scale for profiler byte counters
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.sum(flush, dim=0, out=flush_sink)
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"spmv": res_spmv, "copy_bytes_each_way": src.numel() * 4}), flush=True)
+        return 0
     del flush
     torch.cuda.empty_cache()
 
@@ -173,7 +179,7 @@ def main() -> int:
     ir_ms = statistics.median(step_ms)
     log(f"IR: {ir_ms:.2f} ms/solve ({outer_its} outer, {inner_its} inner), relres {ir_relres:.2e}")
 
-    # ---- CPU baseline: compiled reference Jacobi-PCG, bounded sample -------
+    # ---- CPU baseline: the compiled reference's Jacobi-PCG, full solve -----
     cpu = None
     if not args.no_cpu:
         try:
@@ -181,21 +187,31 @@ def main() -> int:
             from oracle import CG as RCG, Matrix, Reference
             Rf = Reference()
             M = Matrix.of(A64)
-            sample_iters = 60
-            t0 = time.perf_counter()
-            out = Rf.solve(RCG, M, pr.b, np.zeros(n), pc="jacobi", weight=1.0, maxIter=sample_iters,
+            out = Rf.solve(RCG, M, pr.b, np.zeros(n), pc="jacobi", weight=1.0, maxIter=20000,
                            relTol=1e-8, absTol=0.0)
-            cpu_s = time.perf_counter() - t0
-            per_iter = cpu_s * 1e3 / out["numIter"]
-            cpu = {"value": per_iter * rcg.numIter, "unit": "ms", "cores": 1, "kind": "reference",
-                   "sample": (f"reference CG::Solve (oracle/_ref, -O2, 1 thread) on the full 128^3 "
-                              f"system, first {out['numIter']} Jacobi-PCG iterations in "
-                              f"{cpu_s:.1f} s; time-to-rtol projected to the "
-                              f"{rcg.numIter} iterations the solve needs")}
-            log(f"CPU reference: {per_iter:.1f} ms/iter -> {cpu['value']:.0f} ms projected")
+            cpu = {"value": round(out["solve_ms"], 1), "unit": "ms", "cores": 1,
+                   "kind": "reference",
+                   "sample": (f"reference CG::Solve with a Jacobi preconditioner (oracle/_ref, "
+                              f"-O2 -march=x86-64-v3, OpenMP off, 1 thread) on the same 128^3 "
+                              f"system to rtol 1e-8: {out['numIter']} iterations, "
+                              f"steady_clock around Solve only; retcode {out['retcode']}"),
+                   "iters": out["numIter"], "speedup_gpu_fp64_cg": round(out["solve_ms"] / cg["ms"], 1)}
+            log(f"CPU reference: {out['solve_ms']:.0f} ms, {out['numIter']} iters")
         except Exception as e:  # reference library absent on this host
             cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "reference",
                    "sample": f"unavailable: {e}"}
+
+    # HBM bytes per launch of the f64 SpMV, from the committed profiler
+    # counter summary (FETCH_SIZE / WRITE_SIZE runs of `bench.py --spmv-only`,
+    # scaled by a copy of known size); null when it was taken on another
+    # column-index format
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_spmv_f64_poisson3d_128_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("index_bytes") == A.index_bytes and tr.get("nnz") == nnz:
+            traffic = tr["traffic_bytes"]
 
     f64 = res_spmv["f64"]
     line = {
@@ -224,7 +240,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": "k_spmv f64 (MAT::Apply)",
                      "achieved": round(f64["GBs_flushed"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(f64["GBs_flushed"] / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic, "algorithmic_bytes": f64["bytes"],
+                     "index_bytes": A.index_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1,6 +1,7 @@
 // fx_csr.hip -- CSRx matrices in HBM: creation, SpMV (MAT::Apply),
 // residual (MAT::Residual), diagonal (MAT::GetDiag) and transpose
 // (MAT::TransInPlace), src/MAT.cxx.
+#include <cstdlib>
 #include <cstring>
 
 #include "fx_spmv.hpp"
@@ -62,6 +63,24 @@ __global__ void k_get_diag(const double* __restrict__ va, const uint32_t* __rest
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= n) return;
   d[j] = va ? (T)va[dp[j]] : (T)1;
+}
+
+// cd[k] = ci[k] - r0 for the entries of row block b (one workgroup per
+// block); *bad is set when a delta does not fit in 16 bits.
+__global__ void k_col_delta(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                            const uint32_t* __restrict__ blk, uint32_t nblk,
+                            int16_t* __restrict__ cd, uint32_t* bad) {
+  const uint32_t b = blockIdx.x;
+  if (b >= nblk) return;
+  const uint32_t r0 = blk[2 * b];
+  const uint32_t k0 = blk[2 * b + 1], k1 = blk[2 * b + 3];
+  bool fits = true;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += kThreads) {
+    const int64_t d = (int64_t)ci[k] - (int64_t)r0;
+    fits = fits && d >= -32768 && d <= 32767;
+    cd[k] = (int16_t)d;
+  }
+  if (!fits) *bad = 1u;
 }
 
 __global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, uint32_t n) {
@@ -201,13 +220,41 @@ int csr_alloc(fx_csr A) {
 
 int csr_set_blocks(fx_csr A, const uint32_t* host_rp) {
   uint32_t mx = 0;
-  std::vector<uint32_t> blk = build_row_blocks(A->nrow, host_rp, &mx);
-  A->nblk = (uint32_t)blk.size() - 1;
+  const std::vector<uint32_t> rows = build_row_blocks(A->nrow, host_rp, &mx);
+  A->nblk = (uint32_t)rows.size() - 1;
   A->max_row = mx;
+  std::vector<uint32_t> blk(2 * rows.size());
+  for (size_t b = 0; b < rows.size(); ++b) {
+    blk[2 * b] = rows[b];
+    blk[2 * b + 1] = host_rp[rows[b]];
+  }
   FX_HIP(hipMalloc(&A->blk, sizeof(uint32_t) * blk.size()));
+  hipStream_t st = A->ctx->stream;
   FX_HIP(hipMemcpyAsync(A->blk, blk.data(), sizeof(uint32_t) * blk.size(), hipMemcpyHostToDevice,
-                        A->ctx->stream));
-  FX_HIP(hipStreamSynchronize(A->ctx->stream));
+                        st));
+  // 2-byte column deltas when every block admits them (FASPXX_FX_COL32=1 keeps u32 columns)
+  const char* e = std::getenv("FASPXX_FX_COL32");
+  if (A->nnz && A->nblk && !(e && e[0] == '1')) {
+    uint32_t* bad = nullptr;
+    FX_HIP(hipMalloc(&A->cd, sizeof(int16_t) * A->nnz));
+    FX_HIP(hipMalloc(&bad, sizeof(uint32_t)));
+    uint32_t hbad = 0;
+    hipError_t err = hipMemsetAsync(bad, 0, sizeof(uint32_t), st);
+    if (err == hipSuccess) {
+      hipLaunchKernelGGL(k_col_delta, dim3(A->nblk), dim3(kThreads), 0, st, A->rp, A->ci, A->blk,
+                         A->nblk, A->cd, bad);
+      err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(&hbad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    (void)hipFree(bad);
+    if (err != hipSuccess || hbad) {
+      (void)hipFree(A->cd);
+      A->cd = nullptr;
+      FX_HIP(err);
+    }
+  }
+  FX_HIP(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -220,6 +267,7 @@ void csr_free(fx_csr A) {
   (void)hipFree(A->va);
   (void)hipFree(A->va32);
   (void)hipFree(A->blk);
+  (void)hipFree(A->cd);
   (void)hipFree(A->src);
   delete A;
 }
@@ -321,6 +369,13 @@ int fx_csr_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, uint32_t nnz, const 
 
 int fx_csr_destroy(fx_csr A) {
   csr_free(A);
+  return 0;
+}
+
+int fx_csr_index_bytes(fx_csr A, int* bytes) {
+  clear_error();
+  FX_CHECK(A && bytes, FX_ERROR_INPUT_PAR, "null argument");
+  *bytes = A->cd ? 2 : 4;
   return 0;
 }
 

@@ -72,19 +72,25 @@ struct EpiCgQ {  // q = A p and (q, p) -> alpha   (src/CG.cxx:99-119)
   VT* __restrict__ q;
   const VT* __restrict__ p;
   CgState* st;
+  double tmpa = 0.0;      // finisher: loaded in begin()
+  uint32_t numIter = 0;
   __device__ bool skip() const { return st->status != ST_RUN; }
   __device__ __forceinline__ void row(uint32_t r, double v, double (&acc)[1]) const {
     const VT qv = (VT)v;
     q[r] = qv;
     acc[0] = fma((double)qv, (double)p[r], acc[0]);
   }
+  __device__ void begin() {
+    tmpa = st->tmpa;
+    numIter = st->numIter;
+  }
   __device__ void fin(double (&s)[1]) const {
     if (threadIdx.x != 0) return;
     CgState* S = st;
-    S->numIter += 1;
+    S->numIter = numIter + 1;
     S->pq = s[0];
     if (fabs(s[0]) > kCloseZero * kCloseZero) {
-      S->alpha = S->tmpa / s[0];
+      S->alpha = tmpa / s[0];
     } else {
       S->status = ST_HOST;
       S->reason = R_BREAKDOWN;
@@ -153,6 +159,26 @@ __device__ void cg_after_update(CgState* S, const CgPrm& P, const double (&a)[5]
   cg_loop_top(S, P, a[0], hist);
 }
 
+// The streaming kernels below read and write 16-byte packs, two packs per
+// thread per pass (all loads of a pass issued together); the n % W tail
+// elements are done by block 0.  Per element the arithmetic is the
+// reference's (src/CG.cxx:121-126, :223-227).
+template <class VT, bool JAC>
+struct CgUpd {
+  VT a;
+  __device__ __forceinline__ void elem(VT pi, VT qi, VT& xi, VT& ri, VT di,
+                                       double (&acc)[5]) const {
+    xi = fma(a, pi, xi);   // x.AXPY(alpha, p)
+    ri = fma(-a, qi, ri);  // r.AXPY(-alpha, q)
+    const VT zi = JAC ? ri * di : ri;  // Jacobi sweep from z = 0: z = r .* dinv
+    acc[0] = fma((double)ri, (double)ri, acc[0]);
+    acc[1] = fma((double)zi, (double)ri, acc[1]);
+    acc[2] = fmax(acc[2], fabs((double)xi));
+    acc[3] = fma((double)xi, (double)xi, acc[3]);
+    acc[4] = fma((double)pi, (double)pi, acc[4]);
+  }
+};
+
 template <class VT, bool JAC>
 __global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* __restrict__ r,
                                                         const VT* __restrict__ p,
@@ -160,21 +186,44 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* 
                                                         const VT* __restrict__ dinv, uint32_t n,
                                                         const CgState* st, double* partials) {
   if (st->status != ST_RUN) return;
-  const VT a = (VT)st->alpha;
+  const CgUpd<VT, JAC> u{(VT)st->alpha};
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-    const VT pi = p[i], qi = q[i];
-    const VT xi = fma(a, pi, x[i]);   // x.AXPY(alpha, p)
-    const VT ri = fma(-a, qi, r[i]);  // r.AXPY(-alpha, q)
+  using P = Pack16<VT>;
+  constexpr int W = P::W;
+  const uint32_t nc = n / W, stride = gridDim.x * kThreads;
+  for (uint32_t c0 = blockIdx.x * kThreads + threadIdx.x; c0 < nc; c0 += 2 * stride) {
+    const bool two = c0 + stride < nc;
+    const uint32_t c1 = two ? c0 + stride : c0;
+    P p0 = ld16(p, c0), q0 = ld16(q, c0), x0 = ld16(x, c0), r0 = ld16(r, c0);
+    P p1 = ld16(p, c1), q1 = ld16(q, c1), x1 = ld16(x, c1), r1 = ld16(r, c1);
+    P d0{}, d1{};
+    if constexpr (JAC) {
+      d0 = ld16(dinv, c0);
+      d1 = ld16(dinv, c1);
+    }
+    // both packs are computed and stored unconditionally, so that all loads
+    // issue together; when the second is a duplicate of the first (c1 == c0)
+    // it stores the same values again and its sums are dropped
+    double a1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      u.elem(p0.v[w], q0.v[w], x0.v[w], r0.v[w], d0.v[w], acc);
+      u.elem(p1.v[w], q1.v[w], x1.v[w], r1.v[w], d1.v[w], a1);
+    }
+    st16(x, c0, x0);
+    st16(r, c0, r0);
+    st16(x, c1, x1);
+    st16(r, c1, r1);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      acc[k] = !two ? acc[k] : (k == 2 ? fmax(acc[k], a1[k]) : acc[k] + a1[k]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
+    const uint32_t i = nc * W + threadIdx.x;
+    VT xi = x[i], ri = r[i];
+    u.elem(p[i], q[i], xi, ri, JAC ? dinv[i] : (VT)0, acc);
     x[i] = xi;
     r[i] = ri;
-    const VT zi = JAC ? ri * dinv[i] : ri;  // Jacobi sweep from z = 0: z = r .* dinv
-    acc[0] = fma((double)ri, (double)ri, acc[0]);
-    acc[1] = fma((double)zi, (double)ri, acc[1]);
-    acc[2] = fmax(acc[2], fabs((double)xi));
-    acc[3] = fma((double)xi, (double)xi, acc[3]);
-    acc[4] = fma((double)pi, (double)pi, acc[4]);
   }
   block_partials<5, 4u>(acc, partials);
 }
@@ -183,9 +232,13 @@ struct FinCgUpdate {
   CgState* st;
   CgPrm P;
   double* hist;
-  __device__ bool skip() const { return st->status != ST_RUN; }
-  __device__ void fin(double (&s)[5]) const {
-    if (threadIdx.x == 0) cg_after_update(st, P, s, hist);
+  CgState loc;  // loaded in begin(), overlapping the partials' reads
+  __device__ bool skip() const { return loc.status != ST_RUN; }
+  __device__ void begin() { loc = *st; }
+  __device__ void fin(double (&s)[5]) {
+    if (threadIdx.x != 0) return;
+    cg_after_update(&loc, P, s, hist);
+    *st = loc;
   }
 };
 
@@ -196,10 +249,29 @@ __global__ __launch_bounds__(kThreads) void k_cg_pupdate(VT* __restrict__ p,
                                                          const CgState* st) {
   if (st->status != ST_RUN) return;
   const VT beta = (VT)st->beta;
-  const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-    const VT zi = JAC ? r[i] * dinv[i] : r[i];
-    p[i] = fma(beta, p[i], zi);  // p.XPAY(beta, z)
+  using P = Pack16<VT>;
+  constexpr int W = P::W;
+  const uint32_t nc = n / W, stride = gridDim.x * kThreads;
+  for (uint32_t c0 = blockIdx.x * kThreads + threadIdx.x; c0 < nc; c0 += 2 * stride) {
+    const bool two = c0 + stride < nc;
+    const uint32_t c1 = two ? c0 + stride : c0;
+    P p0 = ld16(p, c0), r0 = ld16(r, c0), p1 = ld16(p, c1), r1 = ld16(r, c1);
+    P d0{}, d1{};
+    if constexpr (JAC) {
+      d0 = ld16(dinv, c0);
+      d1 = ld16(dinv, c1);
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      p0.v[w] = fma(beta, p0.v[w], JAC ? r0.v[w] * d0.v[w] : r0.v[w]);  // p.XPAY(beta, z)
+      p1.v[w] = fma(beta, p1.v[w], JAC ? r1.v[w] * d1.v[w] : r1.v[w]);
+    }
+    st16(p, c0, p0);
+    st16(p, c1, p1);  // c1 == c0 stores the same values again
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
+    const uint32_t i = nc * W + threadIdx.x;
+    p[i] = fma(beta, p[i], JAC ? r[i] * dinv[i] : r[i]);
   }
 }
 
@@ -437,7 +509,7 @@ struct CgRun {
     const uint32_t n = W.n;
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
                                                EpiCgQ<VT>{q, p, dstate()}, W.partials)));
-    const unsigned g = stream_blocks(n);
+    const unsigned g = pack_blocks(n, Pack16<VT>::W);
     hipLaunchKernelGGL((k_cg_update<VT, JAC>), dim3(g), dim3(kThreads), 0, st, x, r, p, q, dinv, n,
                        dstate(), W.partials);
     FX_HIP(hipGetLastError());

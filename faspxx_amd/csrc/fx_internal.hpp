@@ -84,10 +84,12 @@ struct fx_vec_s {
   bool owned = false;
 };
 
-// Row-block table of the SpMV kernels: block b covers rows
-// [blk[b], blk[b+1]).  A block either holds at most kBlockRows rows whose
-// entries fit the LDS stage (<= kBlockNnz), or exactly one row longer than
-// that ("long row", reduced by the whole workgroup).
+// Row-block table of the SpMV kernels: nblk + 1 pairs (first row, first
+// entry), so block b covers rows [blk[2b], blk[2b+2]) and entries
+// [blk[2b+1], blk[2b+3]) -- one scalar load gives a block its whole range.
+// A block either holds at most kBlockRows rows whose entries fit the LDS
+// stage (<= kBlockNnz), or exactly one row longer than that ("long row",
+// reduced by the whole workgroup).
 struct fx_csr_s {
   fx_ctx ctx = nullptr;
   uint32_t nrow = 0, mcol = 0, nnz = 0;
@@ -100,6 +102,11 @@ struct fx_csr_s {
   uint32_t* blk = nullptr;
   uint32_t nblk = 0;
   uint32_t max_row = 0;
+  // Column deltas read by the SpMV kernels in place of ci: cd[k] = ci[k] - r0
+  // with r0 the first row of entry k's row block.  Present (non-null) only
+  // when every block's columns lie within [r0 - 32768, r0 + 32767], as for
+  // banded and stencil matrices; 2 bytes per entry instead of 4.
+  int16_t* cd = nullptr;
   // set on a transpose: entry p of this matrix is entry src[p] of its source
   uint32_t* src = nullptr;
 };
@@ -145,6 +152,27 @@ inline unsigned spmv_grid(uint32_t nblk, int /*cus*/) {
 // ---------------------------------------------------------------------------
 // Device building blocks
 // ---------------------------------------------------------------------------
+
+// 16-byte packs for the streaming kernels: 4 floats or 2 doubles per lane
+// per load (dwordx4).  Vectors are hipMalloc'd, so pack c of a vector is
+// 16-byte aligned.
+template <class T>
+struct alignas(16) Pack16 {
+  static constexpr int W = 16 / (int)sizeof(T);
+  T v[W];
+};
+template <class T>
+__device__ __forceinline__ Pack16<T> ld16(const T* __restrict__ p, uint32_t c) {
+  return reinterpret_cast<const Pack16<T>*>(p)[c];
+}
+template <class T>
+__device__ __forceinline__ void st16(T* __restrict__ p, uint32_t c, const Pack16<T>& v) {
+  reinterpret_cast<Pack16<T>*>(p)[c] = v;
+}
+// Grid of the pack-streaming kernels: two packs per thread and pass.
+inline unsigned pack_blocks(uint32_t n, int W) {
+  return stream_blocks((n + W - 1) / W, 2);
+}
 
 // Workgroup id -> logical row block.  Hardware hands workgroup i to XCD
 // i % 8; this maps XCD k to the contiguous logical range [k*G/8, (k+1)*G/8),
@@ -218,35 +246,101 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double* partials)
   }
 }
 
-// Finisher functor: bool skip() const; void fin(double (&sum)[K]) const (all
-// threads of the finisher workgroup call it; it usually acts in thread 0).
-// 1024 threads, each with kFinUnroll independent loads in flight, so the
-// partials (up to ~16K per slot) are read in a few memory latencies.
+// Optional per-thread set-up of kernel functors (e.g. loading a device scalar
+// once): called when the functor has an init() member.
+template <class T, class = void>
+struct has_init : std::false_type {};
+template <class T>
+struct has_init<T, std::void_t<decltype(std::declval<T&>().init())>> : std::true_type {};
+template <class T>
+__device__ __forceinline__ void maybe_init(T& t) {
+  if constexpr (has_init<T>::value) t.init();
+}
+
+// Finisher functor: bool skip() const; void fin(double (&sum)[K]) (all
+// threads of the finisher workgroup call it; it usually acts in thread 0);
+// optionally void begin(), called first, so that loads of solver state
+// overlap with the partials' reads.
+// 1024 threads.  Up to kFinThreads * kFinUnroll partials per slot every
+// load of every slot is issued before the first wait (indices are clamped,
+// the identity substituted after the load), so the whole read costs about
+// one memory latency; larger G falls back to a strided loop.
 constexpr int kFinThreads = 1024;
 constexpr int kFinUnroll = 8;
-template <int K, unsigned MaxMask, class Fin>
-__global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict__ partials,
-                                                        unsigned G, Fin f) {
-  if (f.skip()) return;
-  __shared__ double red[K][kFinThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  double acc[K];
+template <class T, class = void>
+struct has_begin : std::false_type {};
+template <class T>
+struct has_begin<T, std::void_t<decltype(std::declval<T&>().begin())>> : std::true_type {};
+
+// Per-thread part of the finisher for G <= PER * kFinThreads: PER loads per
+// slot, all issued before the first use.
+template <int K, unsigned MaxMask, int PER>
+__device__ __forceinline__ void fin_load(const double* __restrict__ partials, unsigned G, int tid,
+                                         double (&acc)[K]) {
+  double v[K][PER];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const unsigned b = tid + j * kFinThreads;
+      v[k][j] = partials[k * G + (b < G ? b : G - 1)];
+    }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool mx = (MaxMask >> k) & 1u;
     const double ident = mx ? -INFINITY : 0.0;
     double a = ident;
-    for (unsigned base = 0; base < G; base += kFinThreads * kFinUnroll) {
-      double v[kFinUnroll];
 #pragma unroll
-      for (int j = 0; j < kFinUnroll; ++j) {
-        const unsigned b = base + tid + j * kFinThreads;
-        v[j] = b < G ? partials[k * G + b] : ident;
-      }
-#pragma unroll
-      for (int j = 0; j < kFinUnroll; ++j) a = mx ? fmax(a, v[j]) : a + v[j];
+    for (int j = 0; j < PER; ++j) {
+      const double x = (tid + j * kFinThreads < G) ? v[k][j] : ident;
+      a = mx ? fmax(a, x) : a + x;
     }
-    a = mx ? wave_max(a) : wave_sum(a);
+    acc[k] = a;
+  }
+}
+
+template <int K, unsigned MaxMask, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict__ partials,
+                                                        unsigned G, Fin f0) {
+  Fin f = f0;
+  if constexpr (has_begin<Fin>::value) f.begin();
+  __shared__ double red[K][kFinThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double acc[K];
+  if (G <= (unsigned)kFinThreads) {
+    fin_load<K, MaxMask, 1>(partials, G, tid, acc);
+  } else if (G <= 2u * kFinThreads) {
+    fin_load<K, MaxMask, 2>(partials, G, tid, acc);
+  } else if (G <= 4u * kFinThreads) {
+    fin_load<K, MaxMask, 4>(partials, G, tid, acc);
+  } else if (G <= 8u * kFinThreads) {
+    fin_load<K, MaxMask, 8>(partials, G, tid, acc);
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool mx = (MaxMask >> k) & 1u;
+      const double ident = mx ? -INFINITY : 0.0;
+      double a = ident;
+      for (unsigned base = 0; base < G; base += kFinThreads * kFinUnroll) {
+        double v[kFinUnroll];
+#pragma unroll
+        for (int j = 0; j < kFinUnroll; ++j) {
+          const unsigned b = base + tid + j * kFinThreads;
+          v[j] = b < G ? partials[k * G + b] : ident;
+        }
+#pragma unroll
+        for (int j = 0; j < kFinUnroll; ++j) a = mx ? fmax(a, v[j]) : a + v[j];
+      }
+      acc[k] = a;
+    }
+  }
+  // the stop test comes after the loads are issued (a stopped solver's
+  // stale partials are read and dropped), so its latency is not added
+  if (f.skip()) return;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool mx = (MaxMask >> k) & 1u;
+    const double a = mx ? wave_max(acc[k]) : wave_sum(acc[k]);
     if (lane == 0) red[k][wid] = a;
   }
   __syncthreads();
@@ -267,17 +361,6 @@ inline int launch_finish(hipStream_t st, const double* partials, unsigned G, Fin
                      f);
   FX_HIP(hipGetLastError());
   return 0;
-}
-
-// Optional per-thread set-up of kernel functors (e.g. loading a device scalar
-// once): called when the functor has an init() member.
-template <class T, class = void>
-struct has_init : std::false_type {};
-template <class T>
-struct has_init<T, std::void_t<decltype(std::declval<T&>().init())>> : std::true_type {};
-template <class T>
-__device__ __forceinline__ void maybe_init(T& t) {
-  if constexpr (has_init<T>::value) t.init();
 }
 
 // Finisher that stores the reduced slots; slot 0 optionally transformed

@@ -6,6 +6,8 @@
 //   * the block's row pointers are staged in LDS;
 //   * its column indices and values are read once, fully coalesced (non-
 //     temporal: they are streamed, x is what should stay in cache), into LDS;
+//     the columns come as 2-byte deltas to the block's first row when the
+//     matrix has them (banded / stencil matrices), else as u32;
 //   * rows are then reduced with x gathered straight from memory.  While the
 //     block's rows average at most 32 entries each row is summed by one
 //     thread, left to right, in the reference's order (w = v0*x0, then
@@ -29,9 +31,11 @@ template <class MT, class VT>
 struct SpmvIn {
   const uint32_t* __restrict__ rp;
   const uint32_t* __restrict__ ci;
+  const int16_t* __restrict__ cd;  // column deltas to the block's first row, or null
   const MT* __restrict__ va;  // null: structure-only matrix (values are 1)
-  const uint32_t* __restrict__ blk;
+  const uint32_t* __restrict__ blk;  // (first row, first entry) of each block, nblk + 1 pairs
   uint32_t nblk;
+  uint32_t nnz;  // >= 1 whenever a kernel runs
 };
 
 template <class MT, class VT>
@@ -39,11 +43,11 @@ inline SpmvIn<MT, VT> spmv_in(fx_csr A);
 
 template <>
 inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
-  return {A->rp, A->ci, A->has_values ? A->va : nullptr, A->blk, A->nblk};
+  return {A->rp, A->ci, A->cd, A->has_values ? A->va : nullptr, A->blk, A->nblk, A->nnz};
 }
 template <>
 inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
-  return {A->rp, A->ci, A->has_values ? A->va32 : nullptr, A->blk, A->nblk};
+  return {A->rp, A->ci, A->cd, A->has_values ? A->va32 : nullptr, A->blk, A->nblk, A->nnz};
 }
 
 // LDS footprint: row pointers + staged (column, value) pairs.
@@ -70,6 +74,46 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
   return t;
 }
 
+// Stage a row block into LDS: its nr + 1 row pointers (nr <= 256, so at
+// most two per thread; stored relative to k0) and its nz entries [k0, k0 +
+// nz).  The entry loads are unconditional (indices past the block's end are
+// clamped to its last entry) and branch-free, and the row-pointer loads come
+// after them, so all of a thread's loads are in flight before the first
+// wait.  CD: 16-bit column deltas to r0; VA: the matrix has values.
+template <bool CD, bool VA, class MT, class VT>
+__device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, VT>& lds,
+                                            uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
+                                            int tid) {
+  constexpr int kPer = kBlockNnz / kThreads;
+  // last entry index a load may touch: the block's last entry, or for a
+  // block of empty rows the matrix's last entry (nothing is stored then)
+  const uint32_t klast = nz ? k0 + nz - 1 : in.nnz - 1;
+  uint32_t cix[kPer];
+  MT vv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
+    if constexpr (CD)
+      cix[j] = (uint32_t)(int32_t)__builtin_nontemporal_load(&in.cd[k]);
+    else
+      cix[j] = __builtin_nontemporal_load(&in.ci[k]);
+    if constexpr (VA) vv[j] = __builtin_nontemporal_load(&in.va[k]);
+  }
+  const uint32_t i1 = tid + kThreads;
+  const uint32_t rpa = in.rp[r0 + ((uint32_t)tid <= nr ? tid : nr)];
+  const uint32_t rpb = in.rp[r0 + (i1 <= nr ? i1 : nr)];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t k = tid + j * kThreads;
+    if (k < nz) {
+      lds.ci[k] = CD ? (uint32_t)((int32_t)r0 + (int32_t)cix[j]) : cix[j];
+      lds.va[k] = VA ? vv[j] : (MT)1;
+    }
+  }
+  if ((uint32_t)tid <= nr) lds.rp[tid] = rpa - k0;
+  if (i1 <= nr) lds.rp[i1] = rpb - k0;
+}
+
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
 // (a plain load, or a value formed on the fly); B gives b_i for
 // ROW_B_MINUS_AX.  Epi must provide:
@@ -91,15 +135,16 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
   const unsigned bid = xcd_block();
   const int tid = threadIdx.x;
   if (bid < in.nblk) {
-    const uint32_t r0 = in.blk[bid], r1 = in.blk[bid + 1];
-    const uint32_t k0 = in.rp[r0], k1 = in.rp[r1];
+    const uint32_t r0 = in.blk[2 * bid], k0 = in.blk[2 * bid + 1];
+    const uint32_t r1 = in.blk[2 * bid + 2], k1 = in.blk[2 * bid + 3];
     const uint32_t nr = r1 - r0, nz = k1 - k0;
     if (nz > kBlockNnz) {
       // a single long row: strided partial sums, then a workgroup reduction
       double s = 0.0;
       for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
         const double a = in.va ? (double)in.va[k] : 1.0;
-        s = fma(a, (double)xf(in.ci[k]), s);
+        const uint32_t c = in.cd ? (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]) : in.ci[k];
+        s = fma(a, (double)xf(c), s);
       }
       double t[1] = {s};
       block_reduce<1>(t);
@@ -108,25 +153,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         epi.row(r0, v, acc);
       }
     } else {
-      for (uint32_t i = tid; i <= nr; i += kThreads) lds.rp[i] = in.rp[r0 + i] - k0;
-      // Stage: all of a thread's column / value loads are issued together.
-      constexpr int kPer = kBlockNnz / kThreads;
-      uint32_t cix[kPer];
-      MT vv[kPer];
-#pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        const uint32_t k = tid + j * kThreads;
-        cix[j] = k < nz ? __builtin_nontemporal_load(&in.ci[k0 + k]) : 0u;
-        vv[j] = (k < nz && in.va) ? __builtin_nontemporal_load(&in.va[k0 + k]) : (MT)1;
-      }
-#pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        const uint32_t k = tid + j * kThreads;
-        if (k < nz) {
-          lds.ci[k] = cix[j];
-          lds.va[k] = vv[j];
-        }
-      }
+      if (in.cd)
+        in.va ? stage_block<true, true>(in, lds, r0, k0, nr, nz, tid)
+              : stage_block<true, false>(in, lds, r0, k0, nr, nz, tid);
+      else
+        in.va ? stage_block<false, true>(in, lds, r0, k0, nr, nz, tid)
+              : stage_block<false, false>(in, lds, r0, k0, nr, nz, tid);
       __syncthreads();
       const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {

@@ -110,6 +110,7 @@ _SIGS = {
     "fx_csr_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P,
                                 C.POINTER(_P)]),
     "fx_csr_destroy": (C.c_int, [_P]),
+    "fx_csr_index_bytes": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "fx_csr_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                               C.POINTER(C.c_uint32)]),
     "fx_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
@@ -389,6 +390,13 @@ class Csr:
 
     def make_f32(self):
         _chk(load().fx_csr_make_f32(self.h))
+
+    @property
+    def index_bytes(self) -> int:
+        """Bytes per column index read by the SpMV (2: 16-bit block deltas)."""
+        b = C.c_int()
+        _chk(load().fx_csr_index_bytes(self.h, C.byref(b)))
+        return b.value
 
     def Apply(self, v: Vec, w: Vec):
         _chk(load().fx_spmv(self.ctx.h, self.h, v.h, w.h))

@@ -47,6 +47,11 @@ class _Result(C.Structure):
                 ("normInf", C.c_double)]
 
 
+class _RefResult(C.Structure):
+    _fields_ = [("retcode", C.c_int), ("numIter", C.c_uint32), ("norm2", C.c_double),
+                ("normInf", C.c_double), ("solve_ms", C.c_double)]
+
+
 class _RefParams(C.Structure):
     _fields_ = [("maxIter", C.c_uint32), ("minIter", C.c_uint32), ("savIter", C.c_uint32),
                 ("restart", C.c_uint32), ("relTol", C.c_double), ("absTol", C.c_double)]
@@ -222,7 +227,7 @@ class Reference:
             "ref_axpy": (None, [u, d, v, v]), "ref_xpay": (None, [u, d, v, v]),
             "ref_axpby": (None, [u, d, d, v, v]), "ref_waxpby": (None, [u, d, v, d, v, v]),
             "ref_solve": (C.c_int, [C.c_int, u, u, v, v, v, v, C.c_int, u, d,
-                                    C.POINTER(_RefParams), v, v, C.POINTER(_Result)]),
+                                    C.POINTER(_RefParams), v, v, C.POINTER(_RefResult)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -298,9 +303,9 @@ class Reference:
                             P["relTol"], P["absTol"])
         b = _f64(b)
         x = _f64(x0).copy()
-        res = _Result()
+        res = _RefResult()
         self.L.ref_solve(kind, M.nrow, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), _p(M.dp),
                          1 if pc == "jacobi" else 0, pc_iters, weight, C.byref(params), _p(b),
                          _p(x), C.byref(res))
         return {"retcode": res.retcode, "numIter": res.numIter, "norm2": res.norm2,
-                "normInf": res.normInf, "x": x}
+                "normInf": res.normInf, "solve_ms": res.solve_ms, "x": x}

@@ -10,6 +10,7 @@
 // their back-substitution loops never terminate (unsigned `k >= 0`,
 // src/GMRES.cxx:214, src/FGMRES.cxx:209) and index out of bounds.
 
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -56,6 +57,7 @@ struct ref_result {
   int retcode;
   uint32_t numIter;
   double norm2, normInf;
+  double solve_ms;  // steady_clock around Solve only, as example/TestCG.cxx:86-88 times it
 };
 
 void ref_spmv(uint32_t nrow, uint32_t mcol, uint32_t nnz, const uint32_t* rp,
@@ -172,7 +174,9 @@ int ref_solve(int kind, uint32_t nrow, uint32_t nnz, const uint32_t* rp, const u
     configure(s);
     s.SetupPCD(*pcd);
     s.Setup(A);
+    const auto t0 = std::chrono::steady_clock::now();
     rc = s.Solve(bb, xx);
+    res->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     res->numIter = s.GetIterations();
     res->norm2 = s.GetNorm2();
     res->normInf = s.GetInfNorm();
@@ -181,7 +185,9 @@ int ref_solve(int kind, uint32_t nrow, uint32_t nnz, const uint32_t* rp, const u
     configure(s);
     s.SetupPCD(*pcd);
     s.Setup(A);
+    const auto t0 = std::chrono::steady_clock::now();
     rc = s.Solve(bb, xx);
+    res->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     res->numIter = s.GetIterations();
     res->norm2 = s.GetNorm2();
     res->normInf = s.GetInfNorm();
@@ -190,7 +196,9 @@ int ref_solve(int kind, uint32_t nrow, uint32_t nnz, const uint32_t* rp, const u
     configure(s);
     s.SetWeight(weight);
     s.Setup(A);
+    const auto t0 = std::chrono::steady_clock::now();
     rc = s.Solve(bb, xx);
+    res->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     res->numIter = s.GetIterations();
     res->norm2 = s.GetNorm2();
     res->normInf = s.GetInfNorm();

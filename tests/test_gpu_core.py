@@ -107,6 +107,45 @@ def test_spmv_residual_parity(ctx, oracle, problems, cid, scale):
     assert np.array_equal(va, pr.A.values) and np.array_equal(dp, pr.A.diag_ptr)
 
 
+@pytest.mark.parametrize("cid,scale,nbytes", [(1, None, 2), (2, 32, 2), (3, 256, 2), (5, 64, 2),
+                                               (4, 16384, 2), (4, 200000, 4)])
+def test_column_delta_format(ctx, problems, monkeypatch, cid, scale, nbytes):
+    """16-bit column deltas (banded matrices) and u32 columns give the same
+    bits; the power-law matrix's random columns keep u32 once n exceeds the
+    16-bit reach (at n = 16384 every column is within 32K of every row)."""
+    pr = problems(cid, scale)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    monkeypatch.setenv("FASPXX_FX_COL32", "1")
+    A32 = fx.Csr.from_problem(ctx, pr.A)
+    monkeypatch.delenv("FASPXX_FX_COL32")
+    assert A.index_bytes == nbytes and A32.index_bytes == 4
+    x = dev(ctx, np.random.default_rng(7).uniform(-1, 1, pr.A.nrow))
+    w, w32 = fx.Vec(ctx, pr.A.nrow), fx.Vec(ctx, pr.A.nrow)
+    A.Apply(x, w)
+    A32.Apply(x, w32)
+    assert np.array_equal(w.numpy(), w32.numpy())
+
+
+def test_column_delta_fallback(ctx, oracle):
+    """One column 39,999 rows away from its block start: the whole matrix
+    keeps u32 columns, and the SpMV stays bit-exact."""
+    from faspxx_amd import problems as P
+    B = P.poisson2d(200)
+    n = B.nrow
+    rp = B.row_ptr.astype(np.int64)
+    ci = list(B.col_ind[: rp[1]]) + [n - 1] + list(B.col_ind[rp[1]:])
+    va = list(B.values[: rp[1]]) + [-0.5] + list(B.values[rp[1]:])
+    rp2 = np.concatenate([[0], rp[1:] + 1]).astype(np.uint32)
+    ci, va = np.array(ci, np.uint32), np.array(va)
+    dp = oracle.form_diagptr(n, rp2, ci)
+    A = fx.Csr(ctx, n, n, rp2, ci, va, None)
+    assert A.index_bytes == 4
+    x = np.random.default_rng(8).uniform(-1, 1, n)
+    w = fx.Vec(ctx, n)
+    A.Apply(dev(ctx, x), w)
+    assert np.array_equal(w.numpy(), oracle.spmv(Matrix(n, n, rp2, ci, va, dp), x))
+
+
 def test_spmv_f32(ctx, problems):
     pr = problems(2, 32)
     A = fx.Csr.from_problem(ctx, pr.A)
