@@ -36,6 +36,103 @@ def spmv_bytes(n: int, nnz: int, s: int) -> int:
     return (s + 4) * nnz + 4 * (n + 1) + 2 * s * n
 
 
+def spmv_flushed(fx, ctx, A, n, nnz, dt, flush, sink, warmup, reps):
+    """Mean / median of `reps` SpMV launches (after `warmup`), each preceded by
+    a 512 MB read that evicts the Infinity Cache and timed by HIP events
+    attached to the kernel's own dispatch.  Every launch of the kernel is
+    flushed, warm-up ones included, so a profiler's average over the kernel
+    matches the mean."""
+    import torch
+    s = 8 if dt == fx.F64 else 4
+    xv = fx.Vec.from_numpy(ctx, np.random.default_rng(1).uniform(-1, 1, n), dt)
+    yv = fx.Vec(ctx, n, dt)
+    times = []
+    for i in range(warmup + reps):
+        torch.sum(flush, dim=0, out=sink)
+        ms = A.apply_timed(xv, yv)
+        if i >= warmup:
+            times.append(ms)
+    xv.close()
+    yv.close()
+    med, mean = statistics.median(times), statistics.mean(times)
+    byts = spmv_bytes(n, nnz, s)
+    return {"ms_median_flushed": med, "ms_mean_flushed": mean,
+            "GBs_flushed": byts / mean / 1e6, "GBs_median_flushed": byts / med / 1e6,
+            "GFLOPs_flushed": 2 * nnz / mean / 1e6, "bytes": byts, "reps": reps,
+            "index_bytes": A.index_bytes}
+
+
+def relres(pr, x):
+    return float(np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b))
+
+
+def leg_convdiff(fx, P, ctx):
+    """2D upwind convection-diffusion 1024^2: BiCGStab, GMRES(30), FGMRES(30),
+    each Jacobi-preconditioned, to rtol 1e-8 from x0 = 0."""
+    t0 = time.time()
+    pr = P.config(3)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    b = fx.Vec.from_numpy(ctx, pr.b)
+    x = fx.Vec(ctx, pr.A.nrow)
+    pc = fx.Precond(ctx, "jacobi", A, 1.0, 1)
+    out = {"workload": "convdiff2d_1024 (n = %d, nnz = %d)" % (pr.A.nrow, pr.A.nnz)}
+    for kind, name, reps in ((fx.BICGSTAB, "bicgstab", 2), (fx.GMRES, "gmres30", 1),
+                             (fx.FGMRES, "fgmres30", 1)):
+        s = fx.Solver(ctx, kind)
+        s.Setup(A, pc, fx.params(maxIter=20000, relTol=1e-8, absTol=0.0, restart=30))
+        ms = []
+        for _ in range(reps):
+            x.set(0.0)
+            r = s.Solve(b, x)
+            ms.append(r.solve_ms)
+        out[name] = {"ms": round(min(ms), 3), "iters": r.numIter, "retcode": r.retcode,
+                     "relres_fp64": relres(pr, x.numpy()),
+                     "path": "fused graph replay" if kind == fx.BICGSTAB else "per-operation"}
+        log(f"convdiff {name}: {min(ms):.1f} ms, {r.numIter} iters, relres "
+            f"{out[name]['relres_fp64']:.2e}")
+        s.close()
+    out["setup_and_generate_s"] = round(time.time() - t0, 2)
+    return out
+
+
+def leg_powerlaw(fx, P, ctx, flush, sink, warmup, reps):
+    """Power-law SPD M-matrix, n = 2^20: f64 / f32 SpMV, FP64 Jacobi-CG and
+    FP32-inner CG with FP64 refinement, both to rtol 1e-10."""
+    pr = P.config(4)
+    n, nnz = pr.A.nrow, pr.A.nnz
+    A = fx.Csr.from_problem(ctx, pr.A)
+    A.make_f32()
+    out = {"workload": "powerlaw_1048576 (n = %d, nnz = %d)" % (n, nnz)}
+    for dt in (fx.F64, fx.F32):
+        key = "spmv_f64" if dt == fx.F64 else "spmv_f32"
+        out[key] = spmv_flushed(fx, ctx, A, n, nnz, dt, flush, sink, warmup, reps)
+    b = fx.Vec.from_numpy(ctx, pr.b)
+    x = fx.Vec(ctx, n)
+    pc = fx.Precond(ctx, "jacobi", A, 1.0, 1)
+    s = fx.Solver(ctx, fx.CG)
+    s.Setup(A, pc, fx.params(maxIter=20000, relTol=1e-10, absTol=0.0))
+    ms = []
+    for _ in range(2):
+        x.set(0.0)
+        r = s.Solve(b, x)
+        ms.append(r.solve_ms)
+    out["fp64_cg"] = {"ms": round(min(ms), 3), "iters": r.numIter, "relres_fp64": relres(pr, x.numpy())}
+    ir = fx.IR(ctx, A, 1.0, fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
+               fx.params(maxIter=20000, relTol=1e-4, absTol=0.0))
+    ms = []
+    for _ in range(2):
+        x.set(0.0)
+        rr, inner = ir.Solve(b, x)
+        ms.append(rr.solve_ms)
+    out["ir_cg"] = {"ms": round(min(ms), 3), "outer": rr.numIter, "inner": inner,
+                    "relres_fp64": relres(pr, x.numpy())}
+    out["speedup_ir_vs_fp64"] = round(out["fp64_cg"]["ms"] / out["ir_cg"]["ms"], 3)
+    log(f"powerlaw: SpMV f64 {out['spmv_f64']['GBs_flushed']:.0f} GB/s, f32 "
+        f"{out['spmv_f32']['GBs_flushed']:.0f} GB/s; FP64 CG {out['fp64_cg']['ms']:.2f} ms "
+        f"({out['fp64_cg']['iters']} it), IR {out['ir_cg']['ms']:.2f} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +144,8 @@ def main() -> int:
     ap.add_argument("--spmv-only", action="store_true",
                     help="standalone SpMV and a 256 MiB copy only (for profiler counter runs)")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the convection-diffusion and power-law legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -79,41 +178,19 @@ def main() -> int:
     setup_s = time.time() - t0
     log(f"uploaded + row-blocked in {setup_s:.2f}s")
 
-    def ev():
-        return torch.cuda.Event(enable_timing=True)
-
     # ---- standalone SpMV, Infinity Cache flushed between reps ------------
     # Flush by READING 512 MB (twice the 256 MB Infinity Cache): a write-based
     # flush would leave dirty lines whose write-back lands inside the timed SpMV.
     flush = torch.ones(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
     flush_sink = torch.empty((), dtype=torch.float32, device=dev)
     res_spmv = {}
-    for dt, s in ((fx.F64, 8), (fx.F32, 4)):
-        xv = fx.Vec.from_numpy(ctx, np.random.default_rng(1).uniform(-1, 1, n),
-                               dt)
-        yv = fx.Vec(ctx, n, dt)
-        # every launch of this kernel is flushed (warm-up ones included), so
-        # a profiler's average over the kernel matches the mean below
-        times = []
-        for i in range(args.spmv_warmup + args.spmv_reps):
-            torch.sum(flush, dim=0, out=flush_sink)
-            e0, e1 = ev(), ev()
-            e0.record(stream)
-            A.Apply(xv, yv)
-            e1.record(stream)
-            e1.synchronize()
-            if i >= args.spmv_warmup:
-                times.append(e0.elapsed_time(e1))
-        med, mean = statistics.median(times), statistics.mean(times)
-        byts = spmv_bytes(n, nnz, s)
-        res_spmv["f64" if dt == fx.F64 else "f32"] = {
-            "ms_median_flushed": med, "ms_mean_flushed": mean,
-            "GBs_flushed": byts / mean / 1e6, "GBs_median_flushed": byts / med / 1e6,
-            "GFLOPs_flushed": 2 * nnz / mean / 1e6, "bytes": byts, "reps": args.spmv_reps,
-        }
-        log(f"SpMV {'f64' if dt == fx.F64 else 'f32'}: {mean * 1e3:.1f} us mean flushed "
-            f"({byts / mean / 1e6:.0f} GB/s), median {med * 1e3:.1f} us")
-        del xv, yv
+    for dt in (fx.F64, fx.F32):
+        key = "f64" if dt == fx.F64 else "f32"
+        res_spmv[key] = spmv_flushed(fx, ctx, A, n, nnz, dt, flush, flush_sink,
+                                     args.spmv_warmup, args.spmv_reps)
+        r = res_spmv[key]
+        log(f"SpMV {key}: {r['ms_mean_flushed'] * 1e3:.1f} us mean flushed "
+            f"({r['GBs_flushed']:.0f} GB/s), median {r['ms_median_flushed'] * 1e3:.1f} us")
     if args.spmv_only:
         # a plain copy of known size: the scale for profiler byte counters
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -124,8 +201,14 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"spmv": res_spmv, "copy_bytes_each_way": src.numel() * 4}), flush=True)
         return 0
+    extra = {}
+    if not args.no_extra:
+        extra["powerlaw"] = leg_powerlaw(fx, P, ctx, flush, flush_sink, args.spmv_warmup // 2,
+                                         args.spmv_reps // 4)
     del flush
     torch.cuda.empty_cache()
+    if not args.no_extra:
+        extra["convdiff"] = leg_convdiff(fx, P, ctx)
 
     # ---- 50 weighted-Jacobi sweeps, omega = 2/3 --------------------------
     pj = fx.Precond(ctx, "jacobi", A, 2.0 / 3.0, 1)
@@ -243,6 +326,7 @@ def main() -> int:
                      "traffic": traffic, "algorithmic_bytes": f64["bytes"],
                      "index_bytes": A.index_bytes},
         "cpu_baseline": cpu,
+        "other_configs": extra,
     }
     print(json.dumps(line), flush=True)
     return 0

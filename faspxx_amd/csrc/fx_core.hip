@@ -271,6 +271,8 @@ int fx_ctx_destroy(fx_ctx c) {
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->dscal);
   if (c->hpin) (void)hipHostFree(c->hpin);
+  for (auto& e : c->tev)
+    if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;

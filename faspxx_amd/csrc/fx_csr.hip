@@ -274,15 +274,18 @@ void csr_free(fx_csr A) {
 
 }  // namespace
 
-int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype) {
+int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype, hipEvent_t e0,
+                      hipEvent_t e1) {
   if (A->nrow == 0) return 0;
   if (dtype == FX_F64)
     return launch_spmv<double, double, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<double>{(const double*)x},
-                                                      BNone{}, EpiStore<double>{(double*)y}, nullptr);
+                                                      BNone{}, EpiStore<double>{(double*)y}, nullptr,
+                                                      e0, e1);
   FX_CHECK(A->va32 || !A->has_values, FX_ERROR_INPUT_PAR,
            "f32 SpMV needs the f32 value copy (fx_csr_make_f32)");
   return launch_spmv<float, float, ROW_AX, 0, 0u>(A, ctx->stream, XLoad<float>{(const float*)x},
-                                                  BNone{}, EpiStore<float>{(float*)y}, nullptr);
+                                                  BNone{}, EpiStore<float>{(float*)y}, nullptr, e0,
+                                                  e1);
 }
 
 }  // namespace fx
@@ -440,6 +443,22 @@ int fx_spmv(fx_ctx ctx, fx_csr A, fx_vec v, fx_vec w) {
   FX_CHECK(v->dtype == w->dtype, FX_ERROR_INPUT_PAR, "SpMV vector dtypes differ");
   FX_CHECK(v->d != w->d || A->nrow == 0, FX_ERROR_INPUT_PAR, "SpMV in place is not supported");
   return csr_spmv_dispatch(ctx, A, v->d, w->d, v->dtype);
+}
+
+int fx_spmv_timed(fx_ctx ctx, fx_csr A, fx_vec v, fx_vec w, float* ms) {
+  clear_error();
+  FX_CHECK(ctx && A && v && w && ms, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(v->n == A->mcol && w->n == A->nrow, FX_ERROR_NONMATCH_SIZE,
+           "SpMV sizes: A %ux%u, v %u, w %u", A->nrow, A->mcol, v->n, w->n);
+  FX_CHECK(v->dtype == w->dtype, FX_ERROR_INPUT_PAR, "SpMV vector dtypes differ");
+  FX_CHECK(v->d != w->d, FX_ERROR_INPUT_PAR, "SpMV in place is not supported");
+  FX_CHECK(A->nrow > 0, FX_ERROR_INPUT_PAR, "empty matrix: nothing to time");
+  for (auto& e : ctx->tev)
+    if (!e) FX_HIP(hipEventCreate(&e));
+  FX_TRY(csr_spmv_dispatch(ctx, A, v->d, w->d, v->dtype, ctx->tev[0], ctx->tev[1]));
+  FX_HIP(hipEventSynchronize(ctx->tev[1]));
+  FX_HIP(hipEventElapsedTime(ms, ctx->tev[0], ctx->tev[1]));
+  return 0;
 }
 
 int fx_residual(fx_ctx ctx, fx_csr A, fx_vec b, fx_vec x, fx_vec r) {

@@ -3,29 +3,25 @@
 // Jacobi-preconditioned (one sweep, any weight) or unpreconditioned CG,
 // the weighted-Jacobi solver, and the mixed-precision refinement built on
 // the f32 CG.  All scalars stay in a device state block; the reference's
-// per-iteration decisions (src/CG.cxx:82-236, src/Iter.cxx:69-99) run in the
-// finisher of each reduction kernel.  The rare branches (CG's breakdown
+// per-iteration decisions (src/CG.cxx:82-236, src/Iter.cxx:69-99) run on the
+// device, in the kernel that consumes each reduction (CG) or in the
+// reduction's finisher (Jacobi).  The rare branches (CG's breakdown
 // guard, stagnation checks I-II, the false-convergence check III, the
 // safe-guard copy) stop the device loop and are run on the host with the
 // per-operation backend, after which the device loop resumes.
 //
-// CG iteration = 5 kernels, replayed as a hipGraph of kChunk iterations:
-//   K1  q = A p, (q, p)                                  (SpMV + 1 reduction)
-//   F1  alpha = (z, r) / (q, p)                          (finisher, 1 workgroup)
-//   K2  x += alpha p, r -= alpha q, z = D r;  (r,r) (z,r) max|x| (x,x) (p,p)
-//                                                        (stream + 5 reductions)
-//   F2  the reference's end-of-iteration checks, beta     (finisher)
-//   K3  p = beta p + z                                    (stream)
-// Per iteration HBM traffic: the SpMV's matrix stream + 13 vector passes in
-// f64 units (K1: p read, q written; K2: x p r q D read, x r written; K3: r D
-// p read, p written).  Forming p inside K1 instead (saving K3) was measured
-// slower: it triples the gathered loads of the latency-bound SpMV.
+// CG iteration = 3 kernels (see "the three-kernel iteration" below), replayed
+// as a hipGraph of kChunk iterations; the last iteration of a replay copies
+// the state to a pinned host slot.  Per iteration HBM traffic: the SpMV's
+// matrix stream + 13 vector passes (K1: p read, q written; K2: x p r q D
+// read, x r written; K3: r D p read, p written).  Forming p inside K1
+// instead (saving K3) was measured slower: it triples the gathered loads of
+// the latency-bound SpMV.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 
-#include "fx_solver.hpp"
-#include "fx_spmv.hpp"
+#include "fx_fast.hpp"
 
 namespace fx {
 
@@ -36,21 +32,11 @@ bool fast_disabled() {
 
 namespace {
 
-constexpr int kChunk = 8;  // iterations per graph replay
-
-enum { ST_RUN = 0, ST_DONE = 1, ST_HOST = 2 };
-enum { R_BREAKDOWN = 1, R_CHECKS = 2 };
-
 struct CgState {
   double tmpa, pq, alpha, beta, resAbs, resAbsOld, resRel, denAbs, ratio;
   double rr, rz, xinf, xx, pp;
   uint32_t numIter, stagStep, moreStep, histCount;
   int status, reason, err, pad;
-};
-
-struct CgPrm {
-  uint32_t maxIter, minIter, savIter, pad;
-  double relTol, absTol, solStagTol;
 };
 
 // ---- CG kernels -----------------------------------------------------------
@@ -68,34 +54,17 @@ struct EpiBMinusW {  // r = b - (A x), the Apply + XPAY(-1, b) of src/CG.cxx:70-
 };
 
 template <class VT>
-struct EpiCgQ {  // q = A p and (q, p) -> alpha   (src/CG.cxx:99-119)
+struct EpiCgQ {  // q = A p and per-workgroup partials of (q, p)   (src/CG.cxx:99-113)
   VT* __restrict__ q;
   const VT* __restrict__ p;
-  CgState* st;
-  double tmpa = 0.0;      // finisher: loaded in begin()
-  uint32_t numIter = 0;
+  const CgState* st;
   __device__ bool skip() const { return st->status != ST_RUN; }
   __device__ __forceinline__ void row(uint32_t r, double v, double (&acc)[1]) const {
     const VT qv = (VT)v;
     q[r] = qv;
     acc[0] = fma((double)qv, (double)p[r], acc[0]);
   }
-  __device__ void begin() {
-    tmpa = st->tmpa;
-    numIter = st->numIter;
-  }
-  __device__ void fin(double (&s)[1]) const {
-    if (threadIdx.x != 0) return;
-    CgState* S = st;
-    S->numIter = numIter + 1;
-    S->pq = s[0];
-    if (fabs(s[0]) > kCloseZero * kCloseZero) {
-      S->alpha = tmpa / s[0];
-    } else {
-      S->status = ST_HOST;
-      S->reason = R_BREAKDOWN;
-    }
-  }
+  __device__ void fin(double (&)[1]) const {}  // the sum is consumed by k_cg_update_w
 };
 
 // Loop-top bookkeeping (src/CG.cxx:84-93) for iteration `it`.
@@ -113,7 +82,7 @@ __device__ void cg_loop_top(CgState* S, const CgPrm& P, double rr, double* hist)
     }
   }
   if (it >= P.minIter) {
-    hist[it] = S->resAbs;
+    if (hist) hist[it] = S->resAbs;
     S->histCount = it + 1;
   }
 }
@@ -179,44 +148,106 @@ struct CgUpd {
   }
 };
 
+// ---- the three-kernel iteration ---------------------------------------------
+// K1  q = A p, per-workgroup partials of (q, p)               (k_spmv + EpiCgQ)
+// K2  alpha = tmpa / (q, p) from K1's partials, summed by every workgroup in
+//     the same order; x += alpha p, r -= alpha q; partials of
+//     (r,r) (z,r) max|x| (x,x) (p,p)                           (k_cg_update_w)
+// K3  the same sums of K2's partials in every workgroup; the end-of-iteration
+//     logic of src/CG.cxx:130-234 on a private copy of the state (identical in
+//     every workgroup); p = beta p + z unless the loop stops; workgroup 0
+//     writes the new state                                     (k_cg_pupdate_w)
+// No finisher kernels: the reductions are consumed where they are needed.
+// The state is double-buffered by the iteration's parity within a replay
+// (iteration c reads buffer c & 1 and writes buffer (c + 1) & 1), so that no
+// workgroup of an iteration can see a state written during it; a stopping
+// iteration writes both buffers.
+struct CgSide {  // written by K2's workgroup 0 for K3
+  double pq, alpha;
+  int breakdown, pad;
+};
+
+// One pass of K2 over packs c and c + stride (loaded before use).
 template <class VT, bool JAC>
-__global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* __restrict__ r,
-                                                        const VT* __restrict__ p,
-                                                        const VT* __restrict__ q,
-                                                        const VT* __restrict__ dinv, uint32_t n,
-                                                        const CgState* st, double* partials) {
-  if (st->status != ST_RUN) return;
-  const CgUpd<VT, JAC> u{(VT)st->alpha};
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  using P = Pack16<VT>;
-  constexpr int W = P::W;
-  const uint32_t nc = n / W, stride = gridDim.x * kThreads;
-  for (uint32_t c0 = blockIdx.x * kThreads + threadIdx.x; c0 < nc; c0 += 2 * stride) {
-    const bool two = c0 + stride < nc;
-    const uint32_t c1 = two ? c0 + stride : c0;
-    P p0 = ld16(p, c0), q0 = ld16(q, c0), x0 = ld16(x, c0), r0 = ld16(r, c0);
-    P p1 = ld16(p, c1), q1 = ld16(q, c1), x1 = ld16(x, c1), r1 = ld16(r, c1);
-    P d0{}, d1{};
+struct UpdPass {
+  Pack16<VT> p0, q0, x0, r0, d0, p1, q1, x1, r1, d1;
+  uint32_t c0, c1;
+  bool one, two;
+  __device__ __forceinline__ void load(const VT* __restrict__ p, const VT* __restrict__ q,
+                                       const VT* __restrict__ x, const VT* __restrict__ r,
+                                       const VT* __restrict__ dinv, uint32_t c, uint32_t stride,
+                                       uint32_t nc) {
+    one = c < nc;
+    two = c + stride < nc;
+    c0 = one ? c : 0u;
+    c1 = two ? c + stride : c0;
+    p0 = ld16(p, c0);
+    q0 = ld16(q, c0);
+    x0 = ld16(x, c0);
+    r0 = ld16(r, c0);
+    p1 = ld16(p, c1);
+    q1 = ld16(q, c1);
+    x1 = ld16(x, c1);
+    r1 = ld16(r, c1);
     if constexpr (JAC) {
       d0 = ld16(dinv, c0);
       d1 = ld16(dinv, c1);
     }
-    // both packs are computed and stored unconditionally, so that all loads
-    // issue together; when the second is a duplicate of the first (c1 == c0)
-    // it stores the same values again and its sums are dropped
-    double a1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  }
+  __device__ __forceinline__ void run(const CgUpd<VT, JAC>& u, VT* __restrict__ x,
+                                      VT* __restrict__ r, double (&acc)[5]) {
+    constexpr int W = Pack16<VT>::W;
+    double a0[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, a1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      u.elem(p0.v[w], q0.v[w], x0.v[w], r0.v[w], d0.v[w], acc);
+      u.elem(p0.v[w], q0.v[w], x0.v[w], r0.v[w], d0.v[w], a0);
       u.elem(p1.v[w], q1.v[w], x1.v[w], r1.v[w], d1.v[w], a1);
     }
-    st16(x, c0, x0);
-    st16(r, c0, r0);
-    st16(x, c1, x1);
-    st16(r, c1, r1);
+    if (one) {
+      st16(x, c0, x0);
+      st16(r, c0, r0);
+    }
+    if (two) {
+      st16(x, c1, x1);
+      st16(r, c1, r1);
+    }
 #pragma unroll
-    for (int k = 0; k < 5; ++k)
-      acc[k] = !two ? acc[k] : (k == 2 ? fmax(acc[k], a1[k]) : acc[k] + a1[k]);
+    for (int k = 0; k < 5; ++k) {
+      const double b0 = one ? a0[k] : 0.0, b1 = two ? a1[k] : 0.0;
+      acc[k] = k == 2 ? fmax(acc[k], fmax(b0, b1)) : acc[k] + b0 + b1;
+    }
+  }
+};
+
+template <class VT, bool JAC>
+__global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
+    VT* __restrict__ x, VT* __restrict__ r, const VT* __restrict__ p, const VT* __restrict__ q,
+    const VT* __restrict__ dinv, uint32_t n, const CgState* __restrict__ st,
+    const double* __restrict__ p1, unsigned g1, CgSide* __restrict__ side,
+    double* __restrict__ p2) {
+  if (st->status != ST_RUN) return;
+  const double tmpa = st->tmpa;
+  constexpr int W = Pack16<VT>::W;
+  const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
+  const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
+  UpdPass<VT, JAC> pass;  // the first pass is loaded before alpha is known (with nc == 0 the
+  pass.load(p, q, x, r, dinv, cfirst, stride, nc);  // clamped loads read pack 0 of the padded vectors)
+  double pq[1];
+  partials_thread<1, 0u, kWideThreads>(p1, g1, threadIdx.x, pq);
+  partials_combine<1, 0u, kWideThreads>(pq);
+  const bool brk = !(fabs(pq[0]) > kCloseZero * kCloseZero);  // src/CG.cxx:114-119
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    side->pq = pq[0];
+    side->alpha = brk ? 0.0 : tmpa / pq[0];
+    side->breakdown = brk ? 1 : 0;
+  }
+  if (brk) return;  // the same decision in every workgroup
+  const CgUpd<VT, JAC> u{(VT)(tmpa / pq[0])};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  pass.run(u, x, r, acc);  // stores only packs below nc
+  for (uint32_t c = cfirst + 2 * stride; c < nc; c += 2 * stride) {
+    pass.load(p, q, x, r, dinv, c, stride, nc);
+    pass.run(u, x, r, acc);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
     const uint32_t i = nc * W + threadIdx.x;
@@ -225,49 +256,94 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(VT* __restrict__ x, VT* 
     x[i] = xi;
     r[i] = ri;
   }
-  block_partials<5, 4u>(acc, partials);
+  block_partials_n<5, 4u, kWideThreads>(acc, p2);
 }
 
-struct FinCgUpdate {
-  CgState* st;
-  CgPrm P;
-  double* hist;
-  CgState loc;  // loaded in begin(), overlapping the partials' reads
-  __device__ bool skip() const { return loc.status != ST_RUN; }
-  __device__ void begin() { loc = *st; }
-  __device__ void fin(double (&s)[5]) {
-    if (threadIdx.x != 0) return;
-    cg_after_update(&loc, P, s, hist);
-    *st = loc;
-  }
-};
-
 template <class VT, bool JAC>
-__global__ __launch_bounds__(kThreads) void k_cg_pupdate(VT* __restrict__ p,
-                                                         const VT* __restrict__ r,
-                                                         const VT* __restrict__ dinv, uint32_t n,
-                                                         const CgState* st) {
-  if (st->status != ST_RUN) return;
-  const VT beta = (VT)st->beta;
-  using P = Pack16<VT>;
-  constexpr int W = P::W;
-  const uint32_t nc = n / W, stride = gridDim.x * kThreads;
-  for (uint32_t c0 = blockIdx.x * kThreads + threadIdx.x; c0 < nc; c0 += 2 * stride) {
-    const bool two = c0 + stride < nc;
-    const uint32_t c1 = two ? c0 + stride : c0;
-    P p0 = ld16(p, c0), r0 = ld16(r, c0), p1 = ld16(p, c1), r1 = ld16(r, c1);
-    P d0{}, d1{};
+struct PPass {
+  Pack16<VT> p0, r0, d0, p1, r1, d1;
+  uint32_t c0, c1;
+  bool one, two;
+  __device__ __forceinline__ void load(const VT* __restrict__ p, const VT* __restrict__ r,
+                                       const VT* __restrict__ dinv, uint32_t c, uint32_t stride,
+                                       uint32_t nc) {
+    one = c < nc;
+    two = c + stride < nc;
+    c0 = one ? c : 0u;
+    c1 = two ? c + stride : c0;
+    p0 = ld16(p, c0);
+    r0 = ld16(r, c0);
+    p1 = ld16(p, c1);
+    r1 = ld16(r, c1);
     if constexpr (JAC) {
       d0 = ld16(dinv, c0);
       d1 = ld16(dinv, c1);
     }
+  }
+  __device__ __forceinline__ void run(VT beta, VT* __restrict__ p) {
+    constexpr int W = Pack16<VT>::W;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       p0.v[w] = fma(beta, p0.v[w], JAC ? r0.v[w] * d0.v[w] : r0.v[w]);  // p.XPAY(beta, z)
       p1.v[w] = fma(beta, p1.v[w], JAC ? r1.v[w] * d1.v[w] : r1.v[w]);
     }
-    st16(p, c0, p0);
-    st16(p, c1, p1);  // c1 == c0 stores the same values again
+    if (one) st16(p, c0, p0);
+    if (two) st16(p, c1, p1);
+  }
+};
+
+template <class VT, bool JAC>
+__global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
+    VT* __restrict__ p, const VT* __restrict__ r, const VT* __restrict__ dinv, uint32_t n,
+    CgState* st_cur, CgState* st_next, CgState* hout, const double* __restrict__ p2,
+    unsigned g2, const CgSide* __restrict__ side, CgPrm P, double* hist) {
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (st_cur->status != ST_RUN) {  // stopped: hand the state on unchanged
+    if (lead) {
+      const CgState S = *st_cur;
+      *st_next = S;
+      if (hout) *hout = S;
+    }
+    return;
+  }
+  constexpr int W = Pack16<VT>::W;
+  const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
+  const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
+  PPass<VT, JAC> pass;  // loaded before beta is known (nc == 0: pack 0 of the padded vectors)
+  pass.load(p, r, dinv, cfirst, stride, nc);
+  double a[5];
+  partials_thread<5, 4u, kWideThreads>(p2, g2, threadIdx.x, a);
+  partials_combine<5, 4u, kWideThreads>(a);
+  __shared__ double sh_beta;
+  __shared__ int sh_run;
+  if (threadIdx.x == 0) {
+    // src/CG.cxx:99-119 (the (q, p) test), then :121-234, on a private copy
+    CgState S = *st_cur;
+    const CgSide sd = *side;
+    S.numIter += 1;
+    S.pq = sd.pq;
+    if (sd.breakdown) {
+      S.status = ST_HOST;
+      S.reason = R_BREAKDOWN;
+    } else {
+      S.alpha = sd.alpha;
+      cg_after_update(&S, P, a, blockIdx.x == 0 ? hist : nullptr);
+    }
+    sh_beta = S.beta;
+    sh_run = S.status == ST_RUN;
+    if (blockIdx.x == 0) {
+      *st_next = S;
+      if (S.status != ST_RUN) *st_cur = S;  // every later iteration of the replay stops
+      if (hout) *hout = S;
+    }
+  }
+  __syncthreads();
+  if (!sh_run) return;
+  const VT beta = (VT)sh_beta;
+  pass.run(beta, p);  // stores only packs below nc
+  for (uint32_t c = cfirst + 2 * stride; c < nc; c += 2 * stride) {
+    pass.load(p, r, dinv, c, stride, nc);
+    pass.run(beta, p);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
     const uint32_t i = nc * W + threadIdx.x;
@@ -425,48 +501,23 @@ __global__ void k_add_f32_to_f64(double* __restrict__ x, const float* __restrict
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// Workspace
-// ---------------------------------------------------------------------------
-struct FastWs {
-  fx_ctx ctx = nullptr;
-  int dt = FX_F64;
-  uint32_t n = 0;
-  void* x = nullptr;  // internal solution / Jacobi ping buffer
-  void* x2 = nullptr; // Jacobi pong buffer
-  void* r = nullptr;
-  void* p = nullptr;
-  void* q = nullptr;
-  void* safe = nullptr;
-  double* partials = nullptr;
-  void* state = nullptr;  // CgState / JacState
-  void* hstate = nullptr; // pinned, 2 slots
-  double* hist = nullptr;
-  uint32_t hist_cap = 0;
-  double* dscal = nullptr;
-  hipGraphExec_t chunk = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  uint32_t n_cg_updates = 0;
-
-  ~FastWs() {
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
-    void* bufs[] = {x, x2, r, p, q, safe, partials, state, hist, dscal};
-    for (void* b : bufs) (void)hipFree(b);
-    if (hstate) (void)hipHostFree(hstate);
-    if (chunk) (void)hipGraphExecDestroy(chunk);
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-  }
-};
-
-namespace {
+CgPrm make_prm(const fx_params& p) {
+  CgPrm c{};
+  c.maxIter = p.maxIter;
+  c.minIter = p.minIter;
+  c.savIter = p.savIter;
+  c.relTol = p.relTol;
+  c.absTol = p.absTol;
+  c.solStagTol = 1e-4 * p.relTol;
+  return c;
+}
 
 int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_t state_bytes,
              uint32_t spmv_blocks) {
   W.ctx = ctx;
   W.dt = dt;
   W.n = n;
-  const size_t vb = (size_t)(n ? n : 1) * (dt == FX_F64 ? 8 : 4);
+  const size_t vb = pack_round(n) * (dt == FX_F64 ? 8 : 4);  // whole 16-byte packs
   FX_HIP(hipMalloc(&W.x, vb));
   FX_HIP(hipMalloc(&W.x2, vb));
   FX_HIP(hipMalloc(&W.r, vb));
@@ -475,8 +526,11 @@ int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_
   FX_HIP(hipMalloc(&W.safe, vb));
   FX_HIP(hipMemsetAsync(W.safe, 0, vb, ctx->stream));
   const size_t np = 8 * (size_t)((spmv_blocks > (uint32_t)kMaxRedBlocks) ? spmv_blocks : kMaxRedBlocks);
-  FX_HIP(hipMalloc(&W.partials, sizeof(double) * np));
-  FX_HIP(hipMalloc(&W.state, state_bytes));
+  W.part2_off = np;
+  FX_HIP(hipMalloc(&W.partials, sizeof(double) * (np + 8 * (size_t)kMaxRedBlocks)));
+  FX_HIP(hipMalloc(&W.state, 2 * state_bytes));  // two parity buffers
+  FX_HIP(hipMemsetAsync(W.state, 0, 2 * state_bytes, ctx->stream));
+  FX_HIP(hipMalloc(&W.side, 64));
   FX_HIP(hipHostMalloc(&W.hstate, 2 * state_bytes, hipHostMallocDefault));
   W.hist_cap = hist_cap + 2;
   FX_HIP(hipMalloc(&W.hist, sizeof(double) * W.hist_cap));
@@ -485,6 +539,8 @@ int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_
   FX_HIP(hipEventCreateWithFlags(&W.ev[1], hipEventDisableTiming));
   return 0;
 }
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // CG
@@ -498,55 +554,68 @@ struct CgRun {
   CgPrm P;
   hipStream_t st;
 
-  CgState* dstate() const { return (CgState*)W.state; }
+  // state buffer of parity c (iteration c of a replay reads c & 1)
+  CgState* dstate(int c = 0) const { return (CgState*)W.state + (c & 1); }
+  double* part1() const { return W.partials; }
+  double* part2() const { return W.partials + W.part2_off; }
 
   template <bool JAC>
-  int launch_iteration_t() {
+  int launch_iteration_t(int c, CgState* hout) {
     VT* p = (VT*)W.p;
     VT* q = (VT*)W.q;
     VT* x = (VT*)W.x;
     VT* r = (VT*)W.r;
     const uint32_t n = W.n;
+    const int cus = W.ctx->sms;
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
-                                               EpiCgQ<VT>{q, p, dstate()}, W.partials)));
-    const unsigned g = pack_blocks(n, Pack16<VT>::W);
-    hipLaunchKernelGGL((k_cg_update<VT, JAC>), dim3(g), dim3(kThreads), 0, st, x, r, p, q, dinv, n,
-                       dstate(), W.partials);
+                                               EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
+                                               nullptr, false)));
+    const unsigned g1 = spmv_grid(A->nblk, cus);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
+    hipLaunchKernelGGL((k_cg_update_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, x, r, p, q,
+                       dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
+                       (CgSide*)W.side, part2());
     FX_HIP(hipGetLastError());
-    FX_TRY((launch_finish<5, 4u>(st, W.partials, g, FinCgUpdate{dstate(), P, W.hist})));
-    hipLaunchKernelGGL((k_cg_pupdate<VT, JAC>), dim3(g), dim3(kThreads), 0, st, p, r, dinv, n,
-                       dstate());
+    hipLaunchKernelGGL((k_cg_pupdate_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, p, r, dinv,
+                       n, dstate(c), dstate(c + 1), hout, (const double*)part2(), g,
+                       (const CgSide*)W.side, P, W.hist);
     FX_HIP(hipGetLastError());
     return 0;
   }
-  int launch_iteration() { return dinv ? launch_iteration_t<true>() : launch_iteration_t<false>(); }
+  int launch_iteration(int c, CgState* hout) {
+    return dinv ? launch_iteration_t<true>(c, hout) : launch_iteration_t<false>(c, hout);
+  }
+  CgState* hslot(int slot) const { return (CgState*)W.hstate + slot; }
 
   // The legacy null stream cannot be captured: replay by direct launches.
   bool use_graph() const { return st != nullptr; }
 
+  // One graph per host slot (the last iteration copies the state there).
   int build_graph() {
-    if (W.chunk || !use_graph()) return 0;
-    hipGraph_t g = nullptr;
-    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int c = 0; c < kChunk && !rc; ++c) rc = launch_iteration();
-    hipError_t e = hipStreamEndCapture(st, &g);
-    if (rc) return rc;
-    FX_HIP(e);
-    e = hipGraphInstantiate(&W.chunk, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    FX_HIP(e);
+    if (W.chunk[0] || !use_graph()) return 0;
+    for (int slot = 0; slot < 2; ++slot) {
+      hipGraph_t g = nullptr;
+      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int c = 0; c < kChunk && !rc; ++c)
+        rc = launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      FX_HIP(e);
+      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      FX_HIP(e);
+    }
     return 0;
   }
 
   int launch_chunk(int slot) {
     if (use_graph()) {
-      FX_HIP(hipGraphLaunch(W.chunk, st));
+      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
     } else {
-      for (int c = 0; c < kChunk; ++c) FX_TRY(launch_iteration());
+      for (int c = 0; c < kChunk; ++c)
+        FX_TRY(launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr));
     }
-    FX_HIP(hipMemcpyAsync((CgState*)W.hstate + slot, W.state, sizeof(CgState),
-                          hipMemcpyDeviceToHost, st));
     FX_HIP(hipEventRecord(W.ev[slot], st));
     return 0;
   }
@@ -752,17 +821,6 @@ struct CgRun {
   }
 };
 
-CgPrm make_prm(const fx_params& p) {
-  CgPrm c{};
-  c.maxIter = p.maxIter;
-  c.minIter = p.minIter;
-  c.savIter = p.savIter;
-  c.relTol = p.relTol;
-  c.absTol = p.absTol;
-  c.solStagTol = 1e-4 * p.relTol;
-  return c;
-}
-
 // ---------------------------------------------------------------------------
 // Jacobi solver
 // ---------------------------------------------------------------------------
@@ -796,20 +854,20 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
     return 0;
   };
   const bool graph = st != nullptr;  // the legacy null stream cannot be captured
-  if (graph && !W.chunk) {
+  if (graph && !W.chunk[0]) {
     hipGraph_t g = nullptr;
     FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const int rc = sweeps();
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc) return rc;
     FX_HIP(e);
-    e = hipGraphInstantiate(&W.chunk, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&W.chunk[0], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     FX_HIP(e);
   }
   auto replay = [&]() -> int {
     if (graph) {
-      FX_HIP(hipGraphLaunch(W.chunk, st));
+      FX_HIP(hipGraphLaunch(W.chunk[0], st));
       return 0;
     }
     return sweeps();
@@ -862,7 +920,7 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
 
 bool fast_eligible(fx_solver S) {
   if (fast_disabled() || S->A->nrow == 0) return false;
-  if (S->type == FX_SOLVER_CG)
+  if (S->type == FX_SOLVER_CG || S->type == FX_SOLVER_BICGSTAB)
     return S->pc->kind == 0 || (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A == S->A);
   if (S->type == FX_SOLVER_JACOBI) return S->pc->A == S->A;
   return false;
@@ -871,7 +929,9 @@ bool fast_eligible(fx_solver S) {
 int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
   if (!S->fast) {
     S->fast = new FastWs();
-    const size_t sb = S->type == FX_SOLVER_JACOBI ? sizeof(JacState) : sizeof(CgState);
+    const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacState)
+                      : S->type == FX_SOLVER_BICGSTAB ? bicgstab_state_bytes()
+                                                      : sizeof(CgState);
     int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
                       spmv_grid(S->A->nblk, S->ctx->sms));
     if (rc) {
@@ -882,6 +942,8 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
   }
   FastWs& W = *S->fast;
   if (S->type == FX_SOLVER_JACOBI) return jacobi_run(S, W, (const double*)b->d, (double*)x->d, res);
+  if (S->type == FX_SOLVER_BICGSTAB)
+    return bicgstab_fast(S, W, (const double*)b->d, (double*)x->d, res);
   CgRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
                     S->ctx->stream};
   return run.run((const double*)b->d, (double*)x->d, res, S->prm, FX_F64);

@@ -73,6 +73,7 @@ struct fx_ctx_s {
   fx_red_ws ws;
   double* dscal = nullptr;  // device scalar slots for standalone reductions
   double* hpin = nullptr;   // pinned host mirror of dscal
+  hipEvent_t tev[2] = {nullptr, nullptr};  // dispatch-attached timing events (fx_spmv_timed)
   int sms = 256;
 };
 
@@ -113,7 +114,8 @@ struct fx_csr_s {
 
 struct fx_pc_s {
   fx_ctx ctx = nullptr;
-  int kind = 0;  // 0 identity, 1 jacobi
+  int kind = 0;  // 0 identity, 1 jacobi, 2 symmetric multicolor Gauss-Seidel
+  fx_mcgs mcgs = nullptr;  // kind 2
   fx_csr A = nullptr;
   double weight = 1.0;
   uint32_t iters = 1;
@@ -127,10 +129,15 @@ namespace fx {
 #ifndef FX_BLOCK_NNZ
 #define FX_BLOCK_NNZ 2048
 #endif
-constexpr uint32_t kBlockRows = kThreads;
+#ifndef FX_BLOCK_ROWS
+#define FX_BLOCK_ROWS 256
+#endif
+constexpr uint32_t kBlockRows = FX_BLOCK_ROWS;  // a multiple of kThreads
+static_assert(kBlockRows % kThreads == 0, "row blocks hold whole passes of the workgroup");
 constexpr uint32_t kBlockNnz = FX_BLOCK_NNZ;
 
-int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype);
+int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int ensure_dinv32(fx_pc pc);
 
 // Host-side launch geometry for streaming kernels over n elements.
@@ -169,6 +176,11 @@ template <class T>
 __device__ __forceinline__ void st16(T* __restrict__ p, uint32_t c, const Pack16<T>& v) {
   reinterpret_cast<Pack16<T>*>(p)[c] = v;
 }
+// Element count of a vector allocation rounded up to whole 16-byte packs of
+// any element type (a multiple of 4 elements, at least 4), so that pack
+// loads never leave the allocation.
+inline size_t pack_round(uint32_t n) { return ((size_t)n + 3) / 4 * 4 + (n ? 0 : 4); }
+
 // Grid of the pack-streaming kernels: two packs per thread and pass.
 inline unsigned pack_blocks(uint32_t n, int W) {
   return stream_blocks((n + W - 1) / W, 2);
@@ -272,9 +284,9 @@ struct has_begin : std::false_type {};
 template <class T>
 struct has_begin<T, std::void_t<decltype(std::declval<T&>().begin())>> : std::true_type {};
 
-// Per-thread part of the finisher for G <= PER * kFinThreads: PER loads per
-// slot, all issued before the first use.
-template <int K, unsigned MaxMask, int PER>
+// Per-thread part of a partials reduction over NT threads for G <= PER * NT:
+// PER loads per slot, all issued before the first use.
+template <int K, unsigned MaxMask, int PER, int NT>
 __device__ __forceinline__ void fin_load(const double* __restrict__ partials, unsigned G, int tid,
                                          double (&acc)[K]) {
   double v[K][PER];
@@ -282,7 +294,7 @@ __device__ __forceinline__ void fin_load(const double* __restrict__ partials, un
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const unsigned b = tid + j * kFinThreads;
+      const unsigned b = tid + j * NT;
       v[k][j] = partials[k * G + (b < G ? b : G - 1)];
     }
 #pragma unroll
@@ -292,40 +304,40 @@ __device__ __forceinline__ void fin_load(const double* __restrict__ partials, un
     double a = ident;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const double x = (tid + j * kFinThreads < G) ? v[k][j] : ident;
+      const double x = (tid + j * NT < G) ? v[k][j] : ident;
       a = mx ? fmax(a, x) : a + x;
     }
     acc[k] = a;
   }
 }
 
-template <int K, unsigned MaxMask, class Fin>
-__global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict__ partials,
-                                                        unsigned G, Fin f0) {
-  Fin f = f0;
-  if constexpr (has_begin<Fin>::value) f.begin();
-  __shared__ double red[K][kFinThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  double acc[K];
-  if (G <= (unsigned)kFinThreads) {
-    fin_load<K, MaxMask, 1>(partials, G, tid, acc);
-  } else if (G <= 2u * kFinThreads) {
-    fin_load<K, MaxMask, 2>(partials, G, tid, acc);
-  } else if (G <= 4u * kFinThreads) {
-    fin_load<K, MaxMask, 4>(partials, G, tid, acc);
-  } else if (G <= 8u * kFinThreads) {
-    fin_load<K, MaxMask, 8>(partials, G, tid, acc);
+// Per-thread sums of the G partials of each of K slots (layout
+// partials[k * G + g]) over a workgroup of NT threads; the same order for
+// every workgroup that calls it, so every caller gets the same bits.
+template <int K, unsigned MaxMask, int NT>
+__device__ __forceinline__ void partials_thread(const double* __restrict__ partials, unsigned G,
+                                                int tid, double (&acc)[K]) {
+  if (G <= (unsigned)NT) {
+    fin_load<K, MaxMask, 1, NT>(partials, G, tid, acc);
+  } else if (G <= 2u * NT) {
+    fin_load<K, MaxMask, 2, NT>(partials, G, tid, acc);
+  } else if (G <= 4u * NT) {
+    fin_load<K, MaxMask, 4, NT>(partials, G, tid, acc);
+  } else if (G <= 8u * NT) {
+    fin_load<K, MaxMask, 8, NT>(partials, G, tid, acc);
+  } else if (G <= 16u * NT) {
+    fin_load<K, MaxMask, 16, NT>(partials, G, tid, acc);
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const bool mx = (MaxMask >> k) & 1u;
       const double ident = mx ? -INFINITY : 0.0;
       double a = ident;
-      for (unsigned base = 0; base < G; base += kFinThreads * kFinUnroll) {
+      for (unsigned base = 0; base < G; base += NT * kFinUnroll) {
         double v[kFinUnroll];
 #pragma unroll
         for (int j = 0; j < kFinUnroll; ++j) {
-          const unsigned b = base + tid + j * kFinThreads;
+          const unsigned b = base + tid + j * NT;
           v[j] = b < G ? partials[k * G + b] : ident;
         }
 #pragma unroll
@@ -334,9 +346,14 @@ __global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict
       acc[k] = a;
     }
   }
-  // the stop test comes after the loads are issued (a stopped solver's
-  // stale partials are read and dropped), so its latency is not added
-  if (f.skip()) return;
+}
+
+// Combine the per-thread values of partials_thread over the workgroup (wave
+// butterflies, then the waves in order); the result is valid in every thread.
+template <int K, unsigned MaxMask, int NT>
+__device__ __forceinline__ void partials_combine(double (&acc)[K]) {
+  __shared__ double red[K][NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool mx = (MaxMask >> k) & 1u;
@@ -349,9 +366,34 @@ __global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict
     const bool mx = (MaxMask >> k) & 1u;
     double a = red[k][0];
 #pragma unroll
-    for (int w = 1; w < kFinThreads / 64; ++w) a = mx ? fmax(a, red[k][w]) : a + red[k][w];
+    for (int w = 1; w < NT / 64; ++w) a = mx ? fmax(a, red[k][w]) : a + red[k][w];
     acc[k] = a;
   }
+  __syncthreads();  // red may be reused by the next call
+}
+
+// block_partials for a workgroup of NT threads.
+template <int K, unsigned MaxMask, int NT>
+__device__ __forceinline__ void block_partials_n(double (&v)[K], double* partials) {
+  partials_combine<K, MaxMask, NT>(v);
+  if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) partials[k * G + blockIdx.x] = v[k];
+  }
+}
+
+template <int K, unsigned MaxMask, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict__ partials,
+                                                        unsigned G, Fin f0) {
+  Fin f = f0;
+  if constexpr (has_begin<Fin>::value) f.begin();
+  double acc[K];
+  partials_thread<K, MaxMask, kFinThreads>(partials, G, threadIdx.x, acc);
+  // the stop test comes after the loads are issued (a stopped solver's
+  // stale partials are read and dropped), so its latency is not added
+  if (f.skip()) return;
+  partials_combine<K, MaxMask, kFinThreads>(acc);
   f.fin(acc);
 }
 

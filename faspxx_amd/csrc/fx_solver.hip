@@ -39,7 +39,8 @@ __global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, uint3
 int ensure_dinv32(fx_pc pc) {
   if (pc->kind != 1 || pc->dinv32) return 0;
   const uint32_t n = pc->A->nrow;
-  FX_HIP(hipMalloc(&pc->dinv32, sizeof(float) * (n ? n : 1)));
+  // padded to whole 16-byte packs (the streaming kernels load packs)
+  FX_HIP(hipMalloc(&pc->dinv32, sizeof(float) * pack_round(n)));
   if (n)
     hipLaunchKernelGGL(k_d2f, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
                        pc->ctx->stream, pc->dinv, pc->dinv32, n);
@@ -652,7 +653,7 @@ int fx_pc_create_jacobi(fx_ctx ctx, fx_csr A, double weight, uint32_t iters, fx_
   p->weight = weight;
   p->iters = iters;
   const uint32_t n = A->nrow;
-  if (hipMalloc(&p->dinv, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+  if (hipMalloc(&p->dinv, sizeof(double) * pack_round(n)) != hipSuccess) {  // whole 16-byte packs
     delete p;
     set_error("hipMalloc failed");
     return FX_ERROR_ALLOC_MEM;
@@ -669,9 +670,28 @@ int fx_pc_create_jacobi(fx_ctx ctx, fx_csr A, double weight, uint32_t iters, fx_
   return 0;
 }
 
+int fx_pc_create_mcgs(fx_ctx ctx, fx_csr A, double weight, uint32_t sweeps, fx_pc* out) {
+  clear_error();
+  FX_CHECK(ctx && A && out, FX_ERROR_INPUT_PAR, "null argument");
+  fx_pc p = new fx_pc_s();
+  p->ctx = ctx;
+  p->kind = 2;
+  p->A = A;
+  p->weight = weight;
+  p->iters = sweeps;
+  const int rc = fx_mcgs_create(ctx, A, weight, &p->mcgs);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
 int fx_pc_destroy(fx_pc p) {
   if (!p) return 0;
   if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+  fx_mcgs_destroy(p->mcgs);
   (void)hipFree(p->dinv);
   (void)hipFree(p->dinv32);
   delete p;
@@ -685,6 +705,7 @@ int fx_pc_apply(fx_ctx ctx, fx_pc pc, fx_vec b, fx_vec x) {
   if (pc->kind == 0) return fx_vec_copy(b, x);
   FX_CHECK(b->n == pc->A->nrow && x->n == pc->A->nrow, FX_ERROR_NONMATCH_SIZE,
            "preconditioner size mismatch");
+  if (pc->kind == 2) return fx_mcgs_sweep(pc->mcgs, b, x, pc->iters, 1);
   fx_vec work = nullptr, dinv = nullptr;
   int rc = fx_vec_create(ctx, b->n, b->dtype, &work);
   if (rc) return rc;

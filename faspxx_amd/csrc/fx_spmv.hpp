@@ -21,6 +21,8 @@
 // and a one-workgroup finisher kernel runs the epilogue's fin().
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "fx_internal.hpp"
 
 namespace fx {
@@ -74,8 +76,8 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
   return t;
 }
 
-// Stage a row block into LDS: its nr + 1 row pointers (nr <= 256, so at
-// most two per thread; stored relative to k0) and its nz entries [k0, k0 +
+// Stage a row block into LDS: its nr + 1 row pointers (nr <= kBlockRows;
+// stored relative to k0) and its nz entries [k0, k0 +
 // nz).  The entry loads are unconditional (indices past the block's end are
 // clamped to its last entry) and branch-free, and the row-pointer loads come
 // after them, so all of a thread's loads are in flight before the first
@@ -99,9 +101,13 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
       cix[j] = __builtin_nontemporal_load(&in.ci[k]);
     if constexpr (VA) vv[j] = __builtin_nontemporal_load(&in.va[k]);
   }
-  const uint32_t i1 = tid + kThreads;
-  const uint32_t rpa = in.rp[r0 + ((uint32_t)tid <= nr ? tid : nr)];
-  const uint32_t rpb = in.rp[r0 + (i1 <= nr ? i1 : nr)];
+  constexpr int kRp = kBlockRows / kThreads + 1;  // row pointers per thread
+  uint32_t rpv[kRp];
+#pragma unroll
+  for (int j = 0; j < kRp; ++j) {
+    const uint32_t i = tid + j * kThreads;
+    rpv[j] = in.rp[r0 + (i <= nr ? i : nr)];
+  }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = tid + j * kThreads;
@@ -110,8 +116,11 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
       lds.va[k] = VA ? vv[j] : (MT)1;
     }
   }
-  if ((uint32_t)tid <= nr) lds.rp[tid] = rpa - k0;
-  if (i1 <= nr) lds.rp[i1] = rpb - k0;
+#pragma unroll
+  for (int j = 0; j < kRp; ++j) {
+    const uint32_t i = tid + j * kThreads;
+    if (i <= nr) lds.rp[i] = rpv[j] - k0;
+  }
 }
 
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
@@ -139,12 +148,23 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
     const uint32_t r1 = in.blk[2 * bid + 2], k1 = in.blk[2 * bid + 3];
     const uint32_t nr = r1 - r0, nz = k1 - k0;
     if (nz > kBlockNnz) {
-      // a single long row: strided partial sums, then a workgroup reduction
+      // a single long row: strided partial sums (4 strides' loads in
+      // flight at a time, summed in stride order), then a workgroup reduction
       double s = 0.0;
-      for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
-        const double a = in.va ? (double)in.va[k] : 1.0;
-        const uint32_t c = in.cd ? (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]) : in.ci[k];
-        s = fma(a, (double)xf(c), s);
+      for (uint32_t kb = k0 + tid; kb < k1; kb += 4 * kThreads) {
+        double a[4], xv[4];
+        uint32_t c[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t k = min(kb + j * kThreads, k1 - 1);
+          a[j] = in.va ? (double)in.va[k] : 1.0;
+          c[j] = in.cd ? (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]) : in.ci[k];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (kb + j * kThreads < k1) s = fma(a[j], xv[j], s);
       }
       double t[1] = {s};
       block_reduce<1>(t);
@@ -162,18 +182,23 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
       __syncthreads();
       const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {
-        if ((uint32_t)tid < nr) {
-          const uint32_t s = lds.rp[tid], e = lds.rp[tid + 1];
-          double t = (MODE == ROW_AX) ? 0.0 : (double)bf(r0 + tid);
-          // chunks of 8: the chunk's gathers are issued before its fma chain
+        for (uint32_t row = tid; row < nr; row += kThreads) {
+          const uint32_t s = lds.rp[row], e = lds.rp[row + 1];
+          double t = (MODE == ROW_AX) ? 0.0 : (double)bf(r0 + row);
+          // chunks of 8: the chunk's LDS reads, then its gathers, are all
+          // issued before its fma chain (indices past the row's end are
+          // clamped to its last entry and their terms dropped)
           for (uint32_t base = s; base < e; base += 8) {
+            uint32_t cj[8];
             double av[8], xv[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-              const uint32_t k = base + j;
-              av[j] = k < e ? (double)lds.va[k] : 0.0;
-              xv[j] = k < e ? (double)xf(lds.ci[k]) : 0.0;
+              const uint32_t k = min(base + j, e - 1);
+              cj[j] = lds.ci[k];
+              av[j] = (double)lds.va[k];
             }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xv[j] = (double)xf(cj[j]);
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
               const uint32_t k = base + j;
@@ -185,15 +210,29 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
               }
             }
           }
-          epi.row(r0 + tid, t, acc);
+          epi.row(r0 + row, t, acc);
         }
       } else {
         const int grp = tid / tpr, lane = tid % tpr, ngrp = kThreads / tpr;
         for (uint32_t rr = grp; rr < nr; rr += ngrp) {
           const uint32_t s = lds.rp[rr], e = lds.rp[rr + 1];
           double t = 0.0;
-          for (uint32_t k = s + lane; k < e; k += tpr)
-            t = fma((double)lds.va[k], (double)xf(lds.ci[k]), t);
+          // 4 strides' gathers in flight at a time, summed in stride order
+          for (uint32_t kb = s + lane; kb < e; kb += 4 * tpr) {
+            double a[4], xv[4];
+            uint32_t c[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const uint32_t k = min(kb + j * tpr, e - 1);
+              a[j] = (double)lds.va[k];
+              c[j] = lds.ci[k];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              if (kb + j * tpr < e) t = fma(a[j], xv[j], t);
+          }
           t = group_sum(t, tpr);
           if (lane == 0) {
             const double v = (MODE == ROW_AX) ? t : (double)bf(r0 + rr) - t;
@@ -232,13 +271,23 @@ struct EpiStore {
   __device__ void fin(double (&)[1]) const {}
 };
 
+// With events e0 / e1 the kernel is launched with them attached to its
+// dispatch (hipExtLaunchKernelGGL), so that they time the kernel alone.
 template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
-inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials) {
+inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finisher = true) {
   const unsigned g = spmv_grid(A->nblk, A->ctx->sms);
-  hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g), dim3(kThreads), 0,
-                     st, spmv_in<MT, VT>(A), xf, bf, epi, partials);
+  if (e0)
+    hipExtLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g),
+                          dim3(kThreads), 0, st, e0, e1, 0, spmv_in<MT, VT>(A), xf, bf, epi,
+                          partials);
+  else
+    hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g), dim3(kThreads),
+                       0, st, spmv_in<MT, VT>(A), xf, bf, epi, partials);
   FX_HIP(hipGetLastError());
-  if constexpr (NRED > 0) return launch_finish<NRED, MAXMASK>(st, partials, g, epi);
+  if constexpr (NRED > 0) {
+    if (finisher) return launch_finish<NRED, MAXMASK>(st, partials, g, epi);
+  }
   return 0;
 }
 

@@ -111,6 +111,7 @@ _SIGS = {
                                 C.POINTER(_P)]),
     "fx_csr_destroy": (C.c_int, [_P]),
     "fx_csr_index_bytes": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "fx_spmv_timed": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_float)]),
     "fx_csr_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                               C.POINTER(C.c_uint32)]),
     "fx_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
@@ -124,6 +125,11 @@ _SIGS = {
     "fx_csr_transpose_update": (C.c_int, [_P, _P, _P]),
     "fx_csr_pattern_grad": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_pc_create_identity": (C.c_int, [_P, C.POINTER(_P)]),
+    "fx_pc_create_mcgs": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
+    "fx_mcgs_create": (C.c_int, [_P, _P, C.c_double, C.POINTER(_P)]),
+    "fx_mcgs_info": (C.c_int, [_P, C.POINTER(C.c_uint32), _P]),
+    "fx_mcgs_sweep": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_int]),
+    "fx_mcgs_destroy": (C.c_int, [_P]),
     "fx_pc_create_jacobi": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
     "fx_pc_destroy": (C.c_int, [_P]),
     "fx_pc_apply": (C.c_int, [_P, _P, _P, _P]),
@@ -401,6 +407,12 @@ class Csr:
     def Apply(self, v: Vec, w: Vec):
         _chk(load().fx_spmv(self.ctx.h, self.h, v.h, w.h))
 
+    def apply_timed(self, v: Vec, w: Vec) -> float:
+        """Apply, timed by events attached to the kernel's dispatch (ms)."""
+        ms = C.c_float()
+        _chk(load().fx_spmv_timed(self.ctx.h, self.h, v.h, w.h, C.byref(ms)))
+        return ms.value
+
     def Residual(self, b: Vec, x: Vec, r: Vec):
         _chk(load().fx_residual(self.ctx.h, self.h, b.h, x.h, r.h))
 
@@ -449,6 +461,8 @@ class Precond:
             _chk(lib.fx_pc_create_identity(ctx.h, C.byref(h)))
         elif kind == "jacobi":
             _chk(lib.fx_pc_create_jacobi(ctx.h, A.h, weight, iters, C.byref(h)))
+        elif kind == "mcgs":  # symmetric multicolor Gauss-Seidel, `iters` sweeps
+            _chk(lib.fx_pc_create_mcgs(ctx.h, A.h, weight, iters, C.byref(h)))
         else:
             raise ValueError(kind)
         self.h, self.ctx, self.A = h, ctx, A
@@ -460,6 +474,37 @@ class Precond:
     def close(self):
         if self.h:
             load().fx_pc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
+
+
+class MCGS:
+    """Multicolor Gauss-Seidel smoother (fx_mcgs_*)."""
+
+    def __init__(self, ctx: Context, A: Csr, weight: float = 1.0):
+        h = C.c_void_p()
+        _chk(load().fx_mcgs_create(ctx.h, A.h, weight, C.byref(h)))
+        self.h, self.ctx, self.A = h, ctx, A
+        ctx._adopt(self)
+
+    def colors(self):
+        nc = C.c_uint32()
+        col = np.empty(self.A.nrow, np.uint32)
+        _chk(load().fx_mcgs_info(self.h, C.byref(nc), _ptr(col)))
+        return nc.value, col
+
+    def sweep(self, b: Vec, x: Vec, sweeps: int = 1, symmetric: bool = False):
+        _chk(load().fx_mcgs_sweep(self.h, b.h, x.h, sweeps, 1 if symmetric else 0))
+
+    def close(self):
+        if self.h:
+            load().fx_mcgs_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -34,6 +34,7 @@ typedef struct fx_csr_s* fx_csr;
 typedef struct fx_pc_s* fx_pc;
 typedef struct fx_solver_s* fx_solver;
 typedef struct fx_ir_s* fx_ir;
+typedef struct fx_mcgs_s* fx_mcgs;
 
 /* Value types of vectors and matrix values. */
 enum { FX_F64 = 0, FX_F32 = 1 };
@@ -189,6 +190,10 @@ int fx_csr_make_f32(fx_csr A);
 /* w = A v  MAT::Apply, src/MAT.cxx:373-452.  f64 vectors use the f64
  * values; f32 vectors use the f32 copy (fx_csr_make_f32). */
 int fx_spmv(fx_ctx ctx, fx_csr A, fx_vec v, fx_vec w);
+/* fx_spmv with HIP events attached to the kernel's own dispatch; waits for
+ * it and returns its duration in *ms.  A measurement aid with no reference
+ * counterpart (the reference times whole solves, example/TestCG.cxx:86-88). */
+int fx_spmv_timed(fx_ctx ctx, fx_csr A, fx_vec v, fx_vec w, float* ms);
 /* r = b - A x, or r = b if ||x||_inf < 1e-20   MAT::Residual, src/MAT.cxx:455-476 */
 int fx_residual(fx_ctx ctx, fx_csr A, fx_vec b, fx_vec x, fx_vec r);
 /* v = diag(A)  MAT::GetDiag, src/MAT.cxx:256-267 */
@@ -215,6 +220,25 @@ int fx_pc_create_jacobi(fx_ctx ctx, fx_csr A, double weight, uint32_t iters, fx_
 int fx_pc_destroy(fx_pc pc);
 /* x <- P(b) with x as the initial guess: Jacobi::Solve, src/Iter.cxx:56-112 */
 int fx_pc_apply(fx_ctx ctx, fx_pc pc, fx_vec b, fx_vec x);
+
+/* ---- Multicolor Gauss-Seidel smoother --------------------------------------
+ * No reference counterpart: FASP++ has no Gauss-Seidel, and a sequential one
+ * has no parallel form, so this separately validated smoother is offered
+ * instead of a drop-in.  Colors: rows in natural order take the smallest
+ * color unused by already-colored neighbours in the pattern of A + A^T
+ * (stencils come out red-black).  One color is one parallel pass:
+ * r_i = b_i - sum_k a_ik x_k (row order), x_i = fma(r_i, weight / a_ii, x_i).
+ * A sweep runs colors 0..C-1, then C-1..0 when symmetric.  f64 only. */
+int fx_mcgs_create(fx_ctx ctx, fx_csr A, double weight, fx_mcgs* out);
+/* Number of colors, and (if color != NULL) the color of each of the nrow rows. */
+int fx_mcgs_info(fx_mcgs s, uint32_t* ncolors, uint32_t* color);
+/* `sweeps` sweeps updating x in place (b and x distinct). */
+int fx_mcgs_sweep(fx_mcgs s, fx_vec b, fx_vec x, uint32_t sweeps, int symmetric);
+int fx_mcgs_destroy(fx_mcgs s);
+/* Preconditioner z = P r: `sweeps` symmetric multicolor Gauss-Seidel sweeps
+ * on A z = r from the z given (the solvers pass z = 0), usable by every
+ * solver in place of Jacobi (solvers with it run the per-operation path). */
+int fx_pc_create_mcgs(fx_ctx ctx, fx_csr A, double weight, uint32_t sweeps, fx_pc* out);
 
 /* ---- solvers --------------------------------------------------------- */
 /* new CG / BiCGStab / GMRES / FGMRES / Jacobi (src/Krylov.cxx:21-40) */

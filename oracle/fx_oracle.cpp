@@ -77,9 +77,12 @@ struct orc_params {
 // Preconditioner: 0 = Identity (src/Iter.cxx:16-20),
 // 1 = Jacobi used with minIter == maxIter == iters (example/TestPCG.cxx:71-76).
 struct orc_pc {
-  int kind;
+  int kind;            // 0 identity, 1 Jacobi, 2 symmetric multicolor Gauss-Seidel
   uint32_t iters;
   const double* dinv;  // omega / diag, formed as in src/Iter.cxx:45-47
+  const uint32_t* color;  // kind 2: coloring (orc_mc_color) and its weight
+  uint32_t ncolors;
+  double weight;
 };
 
 struct orc_result {
@@ -306,6 +309,75 @@ int orc_check_csrx(const orc_csr* A) {
   return count == ndiag ? RC_SUCCESS : RC_INPUT_PAR;
 }
 
+// ---- Multicolor Gauss-Seidel (no reference counterpart) ---------------------
+// The reference has no Gauss-Seidel; the device offers a multicolor one as a
+// separately validated smoother.  This is its definition.
+//
+// Coloring: rows in natural order, each taking the smallest color not used by
+// an already-colored row adjacent to it in the pattern of A + A^T (self loops
+// ignored).  Returns the number of colors.
+uint32_t orc_mc_color(const orc_csr* A, uint32_t* color) {
+  const uint32_t n = A->nrow;
+  // column lists of A (rows k with a_ki != 0), for the A^T half of the pattern
+  std::vector<uint32_t> tp(n + 1, 0), tr(A->nnz);
+  for (uint32_t k = 0; k < A->nnz; ++k)
+    if (A->ci[k] < n) ++tp[A->ci[k] + 1];
+  for (uint32_t i = 0; i < n; ++i) tp[i + 1] += tp[i];
+  std::vector<uint32_t> cur(tp.begin(), tp.end() - 1);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t k = A->rp[i]; k < A->rp[i + 1]; ++k)
+      if (A->ci[k] < n) tr[cur[A->ci[k]]++] = i;
+  const uint32_t kNone = 0xffffffffu;
+  for (uint32_t i = 0; i < n; ++i) color[i] = kNone;
+  std::vector<uint32_t> mark;  // mark[c] == i: color c is taken by a neighbour of i
+  uint32_t ncolors = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    auto take = [&](uint32_t j) {
+      if (j == i || j >= n || color[j] == kNone) return;
+      if (color[j] >= mark.size()) mark.resize(color[j] + 1, kNone);
+      mark[color[j]] = i;
+    };
+    for (uint32_t k = A->rp[i]; k < A->rp[i + 1]; ++k) take(A->ci[k]);
+    for (uint32_t k = tp[i]; k < tp[i + 1]; ++k) take(tr[k]);
+    uint32_t c = 0;
+    while (c < mark.size() && mark[c] == i) ++c;
+    color[i] = c;
+    if (c + 1 > ncolors) ncolors = c + 1;
+  }
+  return ncolors;
+}
+
+// Sweeps of multicolor Gauss-Seidel (SOR with relaxation `weight`): for each
+// color in turn (0..C-1; for a symmetric sweep then C-1..0), every row i of
+// that color is updated from the current x:
+//     r_i = b_i - sum_k a_ik x_k      (row order, t = b - a*x then t -= a*x)
+//     x_i = fma(r_i, dinv_i, x_i)     dinv_i = weight * (1 / a_ii)
+// Rows of one color do not read each other, so the order inside a color does
+// not matter.
+void orc_mcgs(const orc_csr* A, const uint32_t* color, uint32_t ncolors, double weight,
+              const double* b, double* x, uint32_t sweeps, int symmetric) {
+  const uint32_t n = A->nrow;
+  std::vector<double> dinv(n);
+  orc_jacobi_dinv(A, weight, dinv.data());
+  auto pass = [&](uint32_t c) {
+    for (uint32_t i = 0; i < n; ++i) {
+      if (color[i] != c) continue;
+      const uint32_t s = A->rp[i], e = A->rp[i + 1];
+      double t = b[i];
+      if (e > s) {
+        t = b[i] - A->va[s] * x[A->ci[s]];
+        for (uint32_t k = s + 1; k < e; ++k) t -= A->va[k] * x[A->ci[k]];
+      }
+      x[i] = std::fma(t, dinv[i], x[i]);
+    }
+  };
+  for (uint32_t sw = 0; sw < sweeps; ++sw) {
+    for (uint32_t c = 0; c < ncolors; ++c) pass(c);
+    if (symmetric)
+      for (uint32_t c = ncolors; c-- > 0;) pass(c);
+  }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -336,6 +408,10 @@ void apply_pc(const Ctx& c, const double* r, double* z, Vec& work) {
   const uint32_t n = c.n;
   if (c.pc->kind == 0) {
     std::memcpy(z, r, sizeof(double) * n);
+    return;
+  }
+  if (c.pc->kind == 2) {  // no reference counterpart: the device's multicolor GS
+    orc_mcgs(c.A, c.pc->color, c.pc->ncolors, c.pc->weight, r, z, c.pc->iters, 1);
     return;
   }
   for (uint32_t it = 0; it < c.pc->iters; ++it) {

@@ -39,7 +39,8 @@ class _Params(C.Structure):
 
 
 class _Pc(C.Structure):
-    _fields_ = [("kind", C.c_int), ("iters", C.c_uint32), ("dinv", C.c_void_p)]
+    _fields_ = [("kind", C.c_int), ("iters", C.c_uint32), ("dinv", C.c_void_p),
+                ("color", C.c_void_p), ("ncolors", C.c_uint32), ("weight", C.c_double)]
 
 
 class _Result(C.Structure):
@@ -109,6 +110,8 @@ class Oracle:
             "orc_form_diagptr": (None, [u, v, v, v]), "orc_get_diag": (None, [cs, v]),
             "orc_jacobi_dinv": (None, [cs, d, v]), "orc_transpose": (None, [cs, v, v, v, v]),
             "orc_check_csrx": (C.c_int, [cs]),
+            "orc_mc_color": (u, [cs, v]),
+            "orc_mcgs": (None, [cs, v, u, d, v, v, u, C.c_int]),
             "orc_solve": (C.c_int, [C.c_int, cs, C.POINTER(_Pc), C.POINTER(_Params), v, v,
                                     C.POINTER(_Result), v, u]),
         }
@@ -170,6 +173,20 @@ class Oracle:
         self.L.orc_form_diagptr(nrow, _p(rp), _p(ci), _p(dp))
         return dp
 
+    def mc_color(self, M: Matrix):
+        """Greedy natural-order coloring of A + A^T: (ncolors, color[nrow])."""
+        c = np.empty(M.nrow, np.uint32)
+        nc = self.L.orc_mc_color(C.byref(M.c), _p(c))
+        return nc, c
+
+    def mcgs(self, M: Matrix, color, ncolors, weight, b, x0, sweeps=1, symmetric=False):
+        """Multicolor Gauss-Seidel sweeps from x0; returns the new x."""
+        x = _f64(x0).copy()
+        color = _u32(color)
+        self.L.orc_mcgs(C.byref(M.c), _p(color), ncolors, weight, _p(_f64(b)), _p(x), sweeps,
+                        1 if symmetric else 0)
+        return x
+
     def jacobi_dinv(self, M: Matrix, weight):
         d = np.empty(M.nrow)
         self.L.orc_jacobi_dinv(C.byref(M.c), weight, _p(d))
@@ -197,7 +214,11 @@ class Oracle:
         dinv = None
         if pc == "jacobi" or kind == JACOBI:
             dinv = self.jacobi_dinv(M, weight)
-        pcs = _Pc(1 if dinv is not None else 0, pc_iters, _p(dinv))
+        if pc == "mcgs":
+            ncol, color = self.mc_color(M)
+            pcs = _Pc(2, pc_iters, None, _p(color), ncol, weight)
+        else:
+            pcs = _Pc(1 if dinv is not None else 0, pc_iters, _p(dinv), None, 0, weight)
         res = _Result()
         hist = np.zeros(hist_cap) if hist_cap else None
         self.L.orc_solve(kind, C.byref(M.c), C.byref(pcs), C.byref(params), _p(b), _p(x),

@@ -125,15 +125,55 @@ def test_jacobi_converges_unit(ctx):
         assert np.all(np.abs(x - [0.221719, 0.027149, 0.058824]) < 1e-5)
 
 
-@pytest.mark.parametrize("pc", ["jacobi", "identity"])
-def test_bicgstab_parity(ctx, oracle, problems, pc):
-    pr = problems(3, 128)
+@pytest.mark.parametrize("force_ops", [False, True])
+@pytest.mark.parametrize("cid,scale,pc", [(3, 128, "jacobi"), (3, 128, "identity"),
+                                          (1, 64, "jacobi"), (5, 64, "jacobi")])
+def test_bicgstab_parity(ctx, oracle, problems, cid, scale, pc, force_ops):
+    """Fused device loop (default) and per-operation path against the oracle."""
+    pr = problems(cid, scale)
     M = Matrix.of(pr.A)
-    o = oracle.solve(BICGSTAB, M, pr.b, np.zeros(M.nrow), pc=pc, **PRM)
-    res, x, _ = gpu_solve(ctx, pr, fx.BICGSTAB, pc, **PRM)
+    o = oracle.solve(BICGSTAB, M, pr.b, np.zeros(M.nrow), pc=pc, hist_cap=4000, **PRM)
+    res, x, hist = gpu_solve(ctx, pr, fx.BICGSTAB, pc, force_ops=force_ops, **PRM)
     assert res.retcode == o["retcode"] == 0
     assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
     assert rel_err(x, o["x"]) < 1e-6
+    k = min(len(hist), 20)
+    assert k > 5 and rel_err(hist[:k], o["hist"][:k]) < 1e-8
+
+
+def test_bicgstab_x0_maxit_and_zero_rhs(ctx, oracle, problems):
+    pr = problems(3, 64)
+    M = Matrix.of(pr.A)
+    x0 = np.random.default_rng(5).uniform(-1, 1, M.nrow)
+    for prm in [dict(maxIter=13, relTol=1e-12, absTol=0.0), dict(maxIter=40, minIter=3),
+                dict(maxIter=1000, relTol=1e-6, absTol=0.0, savIter=5)]:
+        o = oracle.solve(BICGSTAB, M, pr.b, x0, pc="jacobi", **prm)
+        res, x, _ = gpu_solve(ctx, pr, fx.BICGSTAB, "jacobi", x0=x0, **prm)
+        assert res.retcode == o["retcode"], prm
+        assert abs(res.numIter - o["numIter"]) <= 2, (prm, res.numIter, o["numIter"])
+        assert rel_err(x, o["x"]) < 1e-6, prm
+
+    class Z:
+        A = pr.A
+        b = np.zeros(pr.A.nrow)
+    res, x, _ = gpu_solve(ctx, Z, fx.BICGSTAB, "jacobi", maxIter=100, relTol=1e-8, absTol=0.0)
+    assert res.numIter == 0 and np.all(x == 0)
+
+
+def test_bicgstab_convdiff_full_size(ctx, problems):
+    """The 1024 x 1024 upwind convection-diffusion system (n = 1,048,576),
+    Jacobi-preconditioned, rtol 1e-8: fused loop and per-operation path agree
+    within 2 iterations and both reach the true relative residual."""
+    pr = problems(3, None)
+    out = []
+    for force_ops in (False, True):
+        res, x, _ = gpu_solve(ctx, pr, fx.BICGSTAB, "jacobi", force_ops=force_ops,
+                              maxIter=5000, relTol=1e-8, absTol=0.0)
+        assert res.retcode == 0
+        true_res = np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b)
+        assert true_res < 2e-8, true_res
+        out.append(res.numIter)
+    assert abs(out[0] - out[1]) <= 2, out
 
 
 @pytest.mark.parametrize("left", [0, 1])
