@@ -74,6 +74,7 @@ def build(verbose: bool = False) -> pathlib.Path:
 
 DROPIN = PKG / "libfaspxx_dropin.so"
 DROPIN_TEST = ROOT / "tests" / "cpp" / "dropin_tests"
+READ_TEST = ROOT / "tests" / "cpp" / "readdata_tests"
 
 
 def build_dropin() -> None:
@@ -83,7 +84,8 @@ def build_dropin() -> None:
     common = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}",
               f"-I{PKG / 'host'}"]
     cmd = [cxx, *common, "-shared", "-o", str(DROPIN), str(PKG / "host" / "faspxx_dropin.cpp"),
-           f"-L{PKG}", "-lfaspxx_hip", "-Wl,-rpath,$ORIGIN"]
+           str(PKG / "host" / "faspxx_readdata.cpp"), f"-L{PKG}", "-lfaspxx_hip",
+           "-Wl,-rpath,$ORIGIN"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"drop-in library build failed:\n{p.stderr}")
@@ -93,6 +95,12 @@ def build_dropin() -> None:
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"drop-in test build failed:\n{p.stderr}")
+    cmd = [cxx, *common, "-o", str(READ_TEST), str(ROOT / "tests" / "cpp" / "readdata_tests.cpp"),
+           f"-L{PKG}", "-lfaspxx_dropin", "-lfaspxx_hip",
+           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../faspxx_amd"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"file-reader test build failed:\n{p.stderr}")
 
 
 if __name__ == "__main__":

@@ -33,6 +33,8 @@ inline unsigned wide_blocks(uint32_t n, int W, int cus) {
   return (unsigned)g;
 }
 
+void gmres_ws_free(void* p);
+
 // ---------------------------------------------------------------------------
 // Workspace
 // ---------------------------------------------------------------------------
@@ -55,12 +57,18 @@ struct FastWs {
   uint32_t hist_cap = 0;
   double* dscal = nullptr;
   void* side = nullptr;        // scalars handed between the kernels of an iteration
+  void* gm = nullptr;          // GMRES basis and cycle graphs (fx_gmres.hip)
   size_t part2_off = 0;        // second partials region (doubles)
   hipGraphExec_t chunk[2] = {nullptr, nullptr};  // replay graphs, one per host state slot
   hipEvent_t ev[2] = {nullptr, nullptr};
 
-  ~FastWs() {
+  ~FastWs();
+};
+
+inline FastWs::~FastWs() {
+  {
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    gmres_ws_free(gm);
     void* bufs[] = {x, x2, r, p, q, safe, s, t, partials, state, hist, dscal, side};
     for (void* b : bufs) (void)hipFree(b);
     if (hstate) (void)hipHostFree(hstate);
@@ -69,7 +77,7 @@ struct FastWs {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
   }
-};
+}
 
 // Allocate a workspace: vectors padded to whole 16-byte packs, two partials
 // regions, two parity buffers of state_bytes, a pinned two-slot host mirror.
@@ -80,5 +88,9 @@ int ws_alloc(FastWs& W, fx_ctx ctx, int dt, uint32_t n, uint32_t hist_cap, size_
 // workspace allocated with it.
 size_t bicgstab_state_bytes();
 int bicgstab_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result* res);
+
+// Fused GMRES (fx_gmres.hip), right or left preconditioned.
+size_t gmres_state_bytes();
+int gmres_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result* res);
 
 }  // namespace fx

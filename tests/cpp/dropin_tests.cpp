@@ -6,12 +6,14 @@
 // a 2D Poisson system.  Prints one line per check; exit status = failures.
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "CG.hxx"
 #include "Iter.hxx"
 #include "Krylov.hxx"
 #include "MAT.hxx"
+#include "ReadData.hxx"
 #include "VEC.hxx"
 
 static int g_fail = 0;
@@ -217,7 +219,47 @@ static void test_solvers() {
   }
 }
 
-int main() {
+// ReadMat (src/ReadData.cxx:361-446) into a device MAT, then CG on it:
+// nos7 as stored is its lower triangle, so solve with the normalised
+// fdm_10X10 system, which is SPD.  The data directory is passed in argv[1].
+static void test_readmat(const char* dir) {
+  const std::string f = std::string(dir) + "/fdm_10X10.csr";
+  MAT A;
+  REQUIRE(ReadMat(f.c_str(), A) == SUCCESS);
+  REQUIRE(A.GetRowSize() == 100 && A.GetNNZ() == 460);
+  VEC ones(100, 1.0), y(100);
+  A.Apply(ones, y);
+  USI row, col, nnz;
+  std::vector<USI> rp, ci;
+  std::vector<DBL> va;
+  REQUIRE(ReadCSRx(f.c_str(), row, col, nnz, rp, ci, va) == SUCCESS);
+  for (USI i = 0; i < row; ++i) {
+    double t = va[rp[i]];
+    for (USI k = rp[i] + 1; k < rp[i + 1]; ++k) t += va[k];
+    REQUIRE(y[i] == t);
+  }
+  class CG s;
+  s.SetMaxIter(500);
+  s.SetRelTol(1e-10);
+  s.SetAbsTol(0.0);
+  Identity id;
+  s.SetupPCD(id);
+  s.Setup(A);
+  VEC x(100, 0.0);
+  REQUIRE(s.Solve(y, x) == SUCCESS);
+  VEC e = x;
+  e.AXPY(-1.0, ones);
+  REQUIRE(e.Norm2() < 1e-7);
+  MAT B;
+  REQUIRE(ReadMat((std::string(dir) + "/nos7-.mtx").c_str(), B) == SUCCESS);
+  REQUIRE(B.GetRowSize() == 729 && B.GetNNZ() == 2673);  // the missing (1,1) inserted as 0
+  REQUIRE(B.GetValue(0, 0) == 0.0);
+  REQUIRE(ReadMat((std::string(dir) + "/missing.csr").c_str(), B) == ERROR_OPEN_FILE);
+  std::printf("ok readmat\n");
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1) test_readmat(argv[1]);
   test_vec();
   test_mat();
   test_jacobi();

@@ -176,17 +176,52 @@ def test_bicgstab_convdiff_full_size(ctx, problems):
     assert abs(out[0] - out[1]) <= 2, out
 
 
+@pytest.mark.parametrize("force_ops", [False, True])
 @pytest.mark.parametrize("left", [0, 1])
 @pytest.mark.parametrize("pc", ["jacobi", "identity"])
-def test_gmres_parity(ctx, oracle, problems, pc, left):
+def test_gmres_parity(ctx, oracle, problems, pc, left, force_ops):
+    """Fused cycle graphs (default) and per-operation path against the oracle."""
     pr = problems(3, 64)
     M = Matrix.of(pr.A)
     prm = dict(maxIter=4000, relTol=1e-8, absTol=0.0, leftPrecond=left)
-    o = oracle.solve(GMRES, M, pr.b, np.zeros(M.nrow), pc=pc, **prm)
-    res, x, _ = gpu_solve(ctx, pr, fx.GMRES, pc, **prm)
+    o = oracle.solve(GMRES, M, pr.b, np.zeros(M.nrow), pc=pc, hist_cap=5000, **prm)
+    res, x, hist = gpu_solve(ctx, pr, fx.GMRES, pc, force_ops=force_ops, **prm)
     assert res.retcode == o["retcode"] == 0
     assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
     assert rel_err(x, o["x"]) < 1e-6
+    assert abs(res.norm2 - o["norm2"]) <= 1e-4 * o["norm2"]
+    k = min(len(hist), 40)
+    assert k > 30 and rel_err(hist[:k], o["hist"][:k]) < 1e-8
+
+
+@pytest.mark.parametrize("prm", [dict(maxIter=17, relTol=1e-12), dict(maxIter=200, minIter=12),
+                                 dict(maxIter=500, restart=7, maxRestart=12, minRestart=4)])
+def test_gmres_parameters(ctx, oracle, problems, prm):
+    """maxIter inside a cycle, minIter, and the variable-restart clamping."""
+    pr = problems(3, 48)
+    M = Matrix.of(pr.A)
+    prm = dict(absTol=0.0, **prm)
+    x0 = np.random.default_rng(9).uniform(-1, 1, M.nrow)
+    o = oracle.solve(GMRES, M, pr.b, x0, pc="jacobi", **prm)
+    res, x, _ = gpu_solve(ctx, pr, fx.GMRES, "jacobi", x0=x0, **prm)
+    assert res.retcode == o["retcode"], prm
+    assert abs(res.numIter - o["numIter"]) <= 2, (prm, res.numIter, o["numIter"])
+    assert rel_err(x, o["x"]) < 1e-6, prm
+
+
+def test_gmres_convdiff_full_size(ctx, problems):
+    """GMRES(30), Jacobi right-preconditioned, on the 1024 x 1024 system to
+    rtol 1e-8: the fused and per-operation paths agree within 2 iterations."""
+    pr = problems(3, None)
+    out = []
+    for force_ops in (False, True):
+        res, x, _ = gpu_solve(ctx, pr, fx.GMRES, "jacobi", force_ops=force_ops,
+                              maxIter=20000, relTol=1e-8, absTol=0.0)
+        assert res.retcode == 0
+        true_res = np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b)
+        assert true_res < 2e-8, true_res
+        out.append(res.numIter)
+    assert abs(out[0] - out[1]) <= 2, out
 
 
 def test_fgmres_parity(ctx, oracle, problems):
