@@ -76,10 +76,15 @@ def leg_convdiff(fx, P, ctx):
     x = fx.Vec(ctx, pr.A.nrow)
     pc = fx.Precond(ctx, "jacobi", A, 1.0, 1)
     out = {"workload": "convdiff2d_1024 (n = %d, nnz = %d)" % (pr.A.nrow, pr.A.nnz)}
-    for kind, name, reps in ((fx.BICGSTAB, "bicgstab", 2), (fx.GMRES, "gmres30", 1),
-                             (fx.FGMRES, "fgmres30", 1)):
+    # FGMRES's Jacobi preconditioner starts from the stale Z of the previous
+    # cycle (src/FGMRES.cxx:158-159), and with it the reference's method does
+    # not reach 1e-8 on this system (the oracle stalls near 1e-4 .. 1e-5), so
+    # it runs a fixed budget of 2000 iterations here
+    for kind, name, reps, maxit in ((fx.BICGSTAB, "bicgstab", 2, 20000),
+                                    (fx.GMRES, "gmres30", 2, 20000),
+                                    (fx.FGMRES, "fgmres30_budget2000", 1, 2000)):
         s = fx.Solver(ctx, kind)
-        s.Setup(A, pc, fx.params(maxIter=20000, relTol=1e-8, absTol=0.0, restart=30))
+        s.Setup(A, pc, fx.params(maxIter=maxit, relTol=1e-8, absTol=0.0, restart=30))
         ms = []
         for _ in range(reps):
             x.set(0.0)
@@ -87,7 +92,8 @@ def leg_convdiff(fx, P, ctx):
             ms.append(r.solve_ms)
         out[name] = {"ms": round(min(ms), 3), "iters": r.numIter, "retcode": r.retcode,
                      "relres_fp64": relres(pr, x.numpy()),
-                     "path": "fused graph replay" if kind == fx.BICGSTAB else "per-operation"}
+                     "us_per_iter": round(min(ms) * 1e3 / max(r.numIter, 1), 1),
+                     "path": "per-operation" if kind == fx.FGMRES else "fused graph replay"}
         log(f"convdiff {name}: {min(ms):.1f} ms, {r.numIter} iters, relres "
             f"{out[name]['relres_fp64']:.2e}")
         s.close()

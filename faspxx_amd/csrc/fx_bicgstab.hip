@@ -355,9 +355,8 @@ __global__ __launch_bounds__(kWideThreads) void k_bi_pupdate(
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (st_cur->status != ST_RUN) {  // stopped: hand the state on unchanged
     if (lead) {
-      const BiState S = *st_cur;
-      *st_next = S;
-      if (hout) *hout = S;
+      copy_state(st_next, st_cur);
+      if (hout) copy_state(hout, st_cur);
     }
     return;
   }
@@ -680,12 +679,14 @@ struct BiRun {
     }
 
     BiState s{};
+    // as for CG: a stretch of replays stops within maxIter / kChunk of them,
+    // and there are at most maxIter host steps
     const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
-    uint64_t chunks = 0;
+    uint64_t host_steps = 0;
     for (;;) {
       FX_TRY(launch_chunk(0));
       FX_TRY(launch_chunk(1));
-      chunks += 2;
+      uint64_t chunks = 2;
       int cur = 0;
       for (;;) {
         FX_HIP(hipEventSynchronize(W.ev[cur]));
@@ -695,6 +696,8 @@ struct BiRun {
         ++chunks;
         cur ^= 1;
       }
+      FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
+               "BiCGStab device loop did not stop");
       FX_TRY(read_state(s));
       if (s.status == ST_DONE) break;
       if (host_step(s, vb_, vx_, vr_, vp_, vpt_, vr0s_, vtmp, vsafe, o, PP)) break;

@@ -68,7 +68,7 @@ struct EpiCgQ {  // q = A p and per-workgroup partials of (q, p)   (src/CG.cxx:9
 };
 
 // Loop-top bookkeeping (src/CG.cxx:84-93) for iteration `it`.
-__device__ void cg_loop_top(CgState* S, const CgPrm& P, double rr, double* hist) {
+__device__ __forceinline__ void cg_loop_top(CgState* S, const CgPrm& P, double rr, double* hist) {
   const uint32_t it = S->numIter;
   if (it == P.minIter) {
     const double resAbs = sqrt(rr);
@@ -89,7 +89,7 @@ __device__ void cg_loop_top(CgState* S, const CgPrm& P, double rr, double* hist)
 
 // End-of-iteration logic of src/CG.cxx:130-234 for the common case; any rare
 // branch hands the whole step to the host (which recomputes it from S).
-__device__ void cg_after_update(CgState* S, const CgPrm& P, const double (&a)[5], double* hist) {
+__device__ __forceinline__ void cg_after_update(CgState* S, const CgPrm& P, const double (&a)[5], double* hist) {
   S->rr = a[0];
   S->rz = a[1];
   S->xinf = a[2];
@@ -167,54 +167,48 @@ struct CgSide {  // written by K2's workgroup 0 for K3
   int breakdown, pad;
 };
 
-// One pass of K2 over packs c and c + stride (loaded before use).
+// One pass of K2 over packs c + i * stride, i < kPassPacks (all loaded
+// before use; packs at or past nc are clamped to pack 0, which the padded
+// vectors hold, and their results dropped).
 template <class VT, bool JAC>
 struct UpdPass {
-  Pack16<VT> p0, q0, x0, r0, d0, p1, q1, x1, r1, d1;
-  uint32_t c0, c1;
-  bool one, two;
-  __device__ __forceinline__ void load(const VT* __restrict__ p, const VT* __restrict__ q,
-                                       const VT* __restrict__ x, const VT* __restrict__ r,
-                                       const VT* __restrict__ dinv, uint32_t c, uint32_t stride,
+  Pack16<VT> p[kPassPacks], q[kPassPacks], x[kPassPacks], r[kPassPacks], d[kPassPacks];
+  uint32_t c[kPassPacks];
+  bool ok[kPassPacks];
+  __device__ __forceinline__ void load(const VT* __restrict__ pv, const VT* __restrict__ qv,
+                                       const VT* __restrict__ xv, const VT* __restrict__ rv,
+                                       const VT* __restrict__ dinv, uint32_t c0, uint32_t stride,
                                        uint32_t nc) {
-    one = c < nc;
-    two = c + stride < nc;
-    c0 = one ? c : 0u;
-    c1 = two ? c + stride : c0;
-    p0 = ld16(p, c0);
-    q0 = ld16(q, c0);
-    x0 = ld16(x, c0);
-    r0 = ld16(r, c0);
-    p1 = ld16(p, c1);
-    q1 = ld16(q, c1);
-    x1 = ld16(x, c1);
-    r1 = ld16(r, c1);
-    if constexpr (JAC) {
-      d0 = ld16(dinv, c0);
-      d1 = ld16(dinv, c1);
+#pragma unroll
+    for (int i = 0; i < kPassPacks; ++i) {
+      const uint32_t ci = c0 + i * stride;
+      ok[i] = ci < nc;
+      c[i] = ok[i] ? ci : 0u;
+      p[i] = ld16(pv, c[i]);
+      q[i] = ld16(qv, c[i]);
+      x[i] = ld16(xv, c[i]);
+      r[i] = ld16(rv, c[i]);
+      if constexpr (JAC) d[i] = ld16(dinv, c[i]);
+      else d[i] = Pack16<VT>{};
     }
   }
-  __device__ __forceinline__ void run(const CgUpd<VT, JAC>& u, VT* __restrict__ x,
-                                      VT* __restrict__ r, double (&acc)[5]) {
+  __device__ __forceinline__ void run(const CgUpd<VT, JAC>& u, VT* __restrict__ xv,
+                                      VT* __restrict__ rv, double (&acc)[5]) {
     constexpr int W = Pack16<VT>::W;
-    double a0[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, a1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      u.elem(p0.v[w], q0.v[w], x0.v[w], r0.v[w], d0.v[w], a0);
-      u.elem(p1.v[w], q1.v[w], x1.v[w], r1.v[w], d1.v[w], a1);
-    }
-    if (one) {
-      st16(x, c0, x0);
-      st16(r, c0, r0);
-    }
-    if (two) {
-      st16(x, c1, x1);
-      st16(r, c1, r1);
-    }
+    for (int i = 0; i < kPassPacks; ++i) {
+      double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const double b0 = one ? a0[k] : 0.0, b1 = two ? a1[k] : 0.0;
-      acc[k] = k == 2 ? fmax(acc[k], fmax(b0, b1)) : acc[k] + b0 + b1;
+      for (int w = 0; w < W; ++w) u.elem(p[i].v[w], q[i].v[w], x[i].v[w], r[i].v[w], d[i].v[w], a);
+      if (ok[i]) {
+        st16(xv, c[i], x[i]);
+        st16(rv, c[i], r[i]);
+      }
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const double b = ok[i] ? a[k] : 0.0;
+        acc[k] = k == 2 ? fmax(acc[k], b) : acc[k] + b;
+      }
     }
   }
 };
@@ -245,7 +239,7 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
   const CgUpd<VT, JAC> u{(VT)(tmpa / pq[0])};
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   pass.run(u, x, r, acc);  // stores only packs below nc
-  for (uint32_t c = cfirst + 2 * stride; c < nc; c += 2 * stride) {
+  for (uint32_t c = cfirst + kPassPacks * stride; c < nc; c += kPassPacks * stride) {
     pass.load(p, q, x, r, dinv, c, stride, nc);
     pass.run(u, x, r, acc);
   }
@@ -259,36 +253,39 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
   block_partials_n<5, 4u, kWideThreads>(acc, p2);
 }
 
+// Packs per thread per pass of K3.
+#ifndef FX_PPACKS
+#define FX_PPACKS 4
+#endif
+constexpr int kPPacks = FX_PPACKS;
 template <class VT, bool JAC>
 struct PPass {
-  Pack16<VT> p0, r0, d0, p1, r1, d1;
-  uint32_t c0, c1;
-  bool one, two;
-  __device__ __forceinline__ void load(const VT* __restrict__ p, const VT* __restrict__ r,
-                                       const VT* __restrict__ dinv, uint32_t c, uint32_t stride,
+  Pack16<VT> p[kPPacks], r[kPPacks], d[kPPacks];
+  uint32_t c[kPPacks];
+  bool ok[kPPacks];
+  __device__ __forceinline__ void load(const VT* __restrict__ pv, const VT* __restrict__ rv,
+                                       const VT* __restrict__ dinv, uint32_t c0, uint32_t stride,
                                        uint32_t nc) {
-    one = c < nc;
-    two = c + stride < nc;
-    c0 = one ? c : 0u;
-    c1 = two ? c + stride : c0;
-    p0 = ld16(p, c0);
-    r0 = ld16(r, c0);
-    p1 = ld16(p, c1);
-    r1 = ld16(r, c1);
-    if constexpr (JAC) {
-      d0 = ld16(dinv, c0);
-      d1 = ld16(dinv, c1);
+#pragma unroll
+    for (int i = 0; i < kPPacks; ++i) {
+      const uint32_t ci = c0 + i * stride;
+      ok[i] = ci < nc;
+      c[i] = ok[i] ? ci : 0u;
+      p[i] = ld16(pv, c[i]);
+      r[i] = ld16(rv, c[i]);
+      if constexpr (JAC) d[i] = ld16(dinv, c[i]);
+      else d[i] = Pack16<VT>{};
     }
   }
-  __device__ __forceinline__ void run(VT beta, VT* __restrict__ p) {
+  __device__ __forceinline__ void run(VT beta, VT* __restrict__ pv) {
     constexpr int W = Pack16<VT>::W;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      p0.v[w] = fma(beta, p0.v[w], JAC ? r0.v[w] * d0.v[w] : r0.v[w]);  // p.XPAY(beta, z)
-      p1.v[w] = fma(beta, p1.v[w], JAC ? r1.v[w] * d1.v[w] : r1.v[w]);
+    for (int i = 0; i < kPPacks; ++i) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        p[i].v[w] = fma(beta, p[i].v[w], JAC ? r[i].v[w] * d[i].v[w] : r[i].v[w]);  // p.XPAY(beta, z)
+      if (ok[i]) st16(pv, c[i], p[i]);
     }
-    if (one) st16(p, c0, p0);
-    if (two) st16(p, c1, p1);
   }
 };
 
@@ -300,9 +297,8 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (st_cur->status != ST_RUN) {  // stopped: hand the state on unchanged
     if (lead) {
-      const CgState S = *st_cur;
-      *st_next = S;
-      if (hout) *hout = S;
+      copy_state(st_next, st_cur);
+      if (hout) copy_state(hout, st_cur);
     }
     return;
   }
@@ -341,7 +337,7 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
   if (!sh_run) return;
   const VT beta = (VT)sh_beta;
   pass.run(beta, p);  // stores only packs below nc
-  for (uint32_t c = cfirst + 2 * stride; c < nc; c += 2 * stride) {
+  for (uint32_t c = cfirst + kPPacks * stride; c < nc; c += kPPacks * stride) {
     pass.load(p, r, dinv, c, stride, nc);
     pass.run(beta, p);
   }
@@ -571,7 +567,7 @@ struct CgRun {
                                                EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
                                                nullptr, false)));
     const unsigned g1 = spmv_grid(A->nblk, cus);
-    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
     hipLaunchKernelGGL((k_cg_update_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, x, r, p, q,
                        dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
                        (CgSide*)W.side, part2());
@@ -779,14 +775,16 @@ struct CgRun {
     o.owned.push_back(vs);
 
     CgState s{};
-    // Every replay advances the device loop by kChunk iterations unless it has
-    // stopped, so the loop must stop within maxIter / kChunk replays.
+    // A replay that finds the loop running advances it by kChunk iterations,
+    // so a stretch of replays between host steps stops within maxIter /
+    // kChunk replays; every host step is preceded by at least one iteration,
+    // so there are at most maxIter of them.
     const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
-    uint64_t chunks = 0;
+    uint64_t host_steps = 0;
     for (;;) {
       FX_TRY(launch_chunk(0));
       FX_TRY(launch_chunk(1));
-      chunks += 2;
+      uint64_t chunks = 2;
       int cur = 0;
       for (;;) {
         FX_HIP(hipEventSynchronize(W.ev[cur]));
@@ -797,6 +795,8 @@ struct CgRun {
         ++chunks;
         cur ^= 1;
       }
+      FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
+               "CG device loop did not stop");
       FX_TRY(read_state(s));
       if (s.status == ST_DONE) break;
       // ST_HOST

@@ -20,16 +20,43 @@ struct CgPrm {
 };
 CgPrm make_prm(const fx_params& p);
 
-// Workgroup size of the streaming kernels that consume reductions.
-constexpr int kWideThreads = 512;
+// Workgroup size of the streaming kernels that consume reductions, and how
+// many of their workgroups a CU may hold (each later workgroup re-reads every
+// workgroup's partial sums, so their number stays small).
+#ifndef FX_WIDE_THREADS
+#define FX_WIDE_THREADS 512
+#endif
+#ifndef FX_WIDE_PER_CU
+#define FX_WIDE_PER_CU 1
+#endif
+constexpr int kWideThreads = FX_WIDE_THREADS;
+constexpr int kWidePerCu = FX_WIDE_PER_CU;
 
-// Workgroups of those kernels over n elements: one per CU at most, two packs
-// per thread per pass.
-inline unsigned wide_blocks(uint32_t n, int W, int cus) {
+// Copy a solver state struct (a multiple of 8 bytes) word by word, so that
+// no private copy of it is made in scratch memory.
+template <class T>
+__device__ __forceinline__ void copy_state(T* __restrict__ dst, const T* __restrict__ src) {
+  static_assert(sizeof(T) % 8 == 0, "state structs are whole 8-byte words");
+  const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 8); ++i) d[i] = s[i];
+}
+
+// Packs per thread per pass in the CG update kernels (all loaded before the
+// reduction that the pass waits for).
+#ifndef FX_PASS_PACKS
+#define FX_PASS_PACKS 4
+#endif
+constexpr int kPassPacks = FX_PASS_PACKS;
+
+// Workgroups of those kernels over n elements: kWidePerCu per CU at most,
+// `per` packs per thread per pass.
+inline unsigned wide_blocks(uint32_t n, int W, int cus, int per = 2) {
   const uint64_t packs = ((uint64_t)n + W - 1) / W;
-  uint64_t g = (packs + 2ull * kWideThreads - 1) / (2ull * kWideThreads);
+  uint64_t g = (packs + (uint64_t)per * kWideThreads - 1) / ((uint64_t)per * kWideThreads);
   if (g < 1) g = 1;
-  if (g > (uint64_t)cus) g = (uint64_t)cus;
+  if (g > (uint64_t)cus * kWidePerCu) g = (uint64_t)cus * kWidePerCu;
   return (unsigned)g;
 }
 

@@ -3,7 +3,9 @@ src/BiCGStab.cxx, src/GMRES.cxx, src/FGMRES.cxx, src/Iter.cxx), through the
 C ABI.  Bars: iteration counts within +-2 at the same relTol; solutions and
 final residual norms within the stated relative tolerances; weighted-Jacobi
 sweeps bit-exact (their updates contain no reduction)."""
+import json
 import os
+import pathlib
 
 import numpy as np
 import pytest
@@ -11,6 +13,7 @@ import pytest
 from conftest import rel_err
 from faspxx_amd import fx
 from oracle import BICGSTAB, CG, FGMRES, GMRES, JACOBI, Matrix
+from perturb import ulp_perturbed
 
 pytestmark = pytest.mark.gpu
 
@@ -125,17 +128,39 @@ def test_jacobi_converges_unit(ctx):
         assert np.all(np.abs(x - [0.221719, 0.027149, 0.058824]) < 1e-5)
 
 
+def oracle_band(oracle, kind, M, b, seeds=(1, 2, 3, 4), **kw):
+    """The oracle's iteration count on b and on one-ulp perturbations of b:
+    (its count on b, the lowest and highest count over all of them)."""
+    base = oracle.solve(kind, M, b, np.zeros(M.nrow), **kw)
+    counts = [base["numIter"]] + [
+        oracle.solve(kind, M, ulp_perturbed(b, s), np.zeros(M.nrow), **kw)["numIter"]
+        for s in seeds]
+    return base, min(counts), max(counts)
+
+
 @pytest.mark.parametrize("force_ops", [False, True])
 @pytest.mark.parametrize("cid,scale,pc", [(3, 128, "jacobi"), (3, 128, "identity"),
-                                          (1, 64, "jacobi"), (5, 64, "jacobi")])
+                                          (3, 256, "jacobi"), (3, 256, "identity")])
 def test_bicgstab_parity(ctx, oracle, problems, cid, scale, pc, force_ops):
-    """Fused device loop (default) and per-operation path against the oracle."""
+    """Fused device loop (default) and per-operation path against the oracle,
+    on the nonsymmetric system BiCGStab is meant for.  (On the symmetric
+    Poisson and variable-coefficient systems the reference's P.A BiCGStab is
+    erratic: on the 64 x 64 variable-coefficient system the oracle's own
+    iterate grows to 1e12, so those make no parity case.)
+
+    The count bar is +-2 around the oracle's counts on b and on one-ulp
+    perturbations of b: at 256 x 256 the oracle itself stops after 375 or 378
+    iterations depending on the last bit of a few entries of b, and a
+    parallel dot product changes the rounding by at least as much."""
     pr = problems(cid, scale)
     M = Matrix.of(pr.A)
-    o = oracle.solve(BICGSTAB, M, pr.b, np.zeros(M.nrow), pc=pc, hist_cap=4000, **PRM)
+    prm = dict(pc=pc, **PRM)
+    o, lo, hi = oracle_band(oracle, BICGSTAB, M, pr.b, hist_cap=4000, **prm)
     res, x, hist = gpu_solve(ctx, pr, fx.BICGSTAB, pc, force_ops=force_ops, **PRM)
     assert res.retcode == o["retcode"] == 0
-    assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+    assert lo - 2 <= res.numIter <= hi + 2, (res.numIter, o["numIter"], lo, hi)
+    if scale == 128:
+        assert lo == hi and abs(res.numIter - o["numIter"]) <= 2
     assert rel_err(x, o["x"]) < 1e-6
     k = min(len(hist), 20)
     assert k > 5 and rel_err(hist[:k], o["hist"][:k]) < 1e-8
@@ -162,18 +187,23 @@ def test_bicgstab_x0_maxit_and_zero_rhs(ctx, oracle, problems):
 
 def test_bicgstab_convdiff_full_size(ctx, problems):
     """The 1024 x 1024 upwind convection-diffusion system (n = 1,048,576),
-    Jacobi-preconditioned, rtol 1e-8: fused loop and per-operation path agree
-    within 2 iterations and both reach the true relative residual."""
+    Jacobi-preconditioned, rtol 1e-8: both paths reach the true relative
+    residual, in a count within +-2 of the band of oracle counts recorded in
+    tests/golden/solver_iterations.json (the oracle on b and on four one-ulp
+    perturbations of b; the oracle takes about 90 s per solve at this size,
+    too long to run here)."""
+    gold = json.loads((pathlib.Path(__file__).parent / "golden" / "solver_iterations.json")
+                      .read_text())["bicgstab_convdiff_1024_jacobi_rtol1e-8"]
+    counts = [gold["numIter"]] + gold["numIter_b_ulp_perturbed"]
+    lo, hi = min(counts), max(counts)
     pr = problems(3, None)
-    out = []
     for force_ops in (False, True):
         res, x, _ = gpu_solve(ctx, pr, fx.BICGSTAB, "jacobi", force_ops=force_ops,
                               maxIter=5000, relTol=1e-8, absTol=0.0)
-        assert res.retcode == 0
+        assert res.retcode == gold["retcode"] == 0
         true_res = np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b)
         assert true_res < 2e-8, true_res
-        out.append(res.numIter)
-    assert abs(out[0] - out[1]) <= 2, out
+        assert lo - 2 <= res.numIter <= hi + 2, (force_ops, res.numIter, lo, hi)
 
 
 @pytest.mark.parametrize("force_ops", [False, True])
