@@ -73,34 +73,34 @@ def build(verbose: bool = False) -> pathlib.Path:
 
 
 DROPIN = PKG / "libfaspxx_dropin.so"
-DROPIN_TEST = ROOT / "tests" / "cpp" / "dropin_tests"
-READ_TEST = ROOT / "tests" / "cpp" / "readdata_tests"
+
+
+EXAMPLES = ["TestKrylov"]
 
 
 def build_dropin() -> None:
-    """The FASP++-interface C++ classes (host code over the C ABI) and the
-    C++ port of the reference's unit tests that exercises them."""
+    """The FASP++-interface C++ classes (host code over the C ABI), the C++
+    ports of the reference's unit tests that exercise them, and the example
+    drivers (faspxx_amd/examples)."""
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}",
               f"-I{PKG / 'host'}"]
-    cmd = [cxx, *common, "-shared", "-o", str(DROPIN), str(PKG / "host" / "faspxx_dropin.cpp"),
-           str(PKG / "host" / "faspxx_readdata.cpp"), f"-L{PKG}", "-lfaspxx_hip",
-           "-Wl,-rpath,$ORIGIN"]
+    host = PKG / "host"
+    cmd = [cxx, *common, "-shared", "-o", str(DROPIN), str(host / "faspxx_dropin.cpp"),
+           str(host / "faspxx_readdata.cpp"), str(host / "faspxx_param.cpp"), f"-L{PKG}",
+           "-lfaspxx_hip", "-Wl,-rpath,$ORIGIN"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"drop-in library build failed:\n{p.stderr}")
-    cmd = [cxx, *common, "-o", str(DROPIN_TEST), str(ROOT / "tests" / "cpp" / "dropin_tests.cpp"),
-           f"-L{PKG}", "-lfaspxx_dropin", "-lfaspxx_hip",
-           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../faspxx_amd"]
-    p = subprocess.run(cmd, capture_output=True, text=True)
-    if p.returncode != 0:
-        raise RuntimeError(f"drop-in test build failed:\n{p.stderr}")
-    cmd = [cxx, *common, "-o", str(READ_TEST), str(ROOT / "tests" / "cpp" / "readdata_tests.cpp"),
-           f"-L{PKG}", "-lfaspxx_dropin", "-lfaspxx_hip",
-           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../faspxx_amd"]
-    p = subprocess.run(cmd, capture_output=True, text=True)
-    if p.returncode != 0:
-        raise RuntimeError(f"file-reader test build failed:\n{p.stderr}")
+    progs = [(ROOT / "tests" / "cpp" / f"{t}.cpp", ROOT / "tests" / "cpp" / t, "$ORIGIN/../../faspxx_amd")
+             for t in ("dropin_tests", "readdata_tests", "param_tests")]
+    progs += [(PKG / "examples" / f"{t}.cpp", PKG / "examples" / t, "$ORIGIN/..") for t in EXAMPLES]
+    for src, out, rpath in progs:
+        cmd = [cxx, *common, "-o", str(out), str(src), f"-L{PKG}", "-lfaspxx_dropin",
+               "-lfaspxx_hip", f"-Wl,-rpath,{PKG}", f"-Wl,-rpath,{rpath}"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"{src.name} build failed:\n{p.stderr}")
 
 
 if __name__ == "__main__":

@@ -447,7 +447,15 @@ void SOL::Record(const fx_result& r) {
   numIter = r.numIter;
   norm2 = r.norm2;
   normInf = r.normInf;
-  if (params.verbose > PRINT_NONE && !(params.minIter >= params.maxIter)) {
+  // the closing report of SOL::PrintFinal, src/SOL.cxx:67-96
+  if (params.minIter >= params.maxIter) return;
+  if (params.verbose >= PRINT_MIN) {
+    std::printf("---------------------------------------------\n");
+    if (numIter >= params.maxIter) std::printf("### WARNING: maxIter = %u reached!\n", params.maxIter);
+    std::printf("Number of iterations : %u\nNorm2 of residual    : %.5e\n"
+                "NormInf of residual  : %.5e\n",
+                numIter, norm2, normInf);
+  } else if (params.verbose > PRINT_NONE) {
     std::printf("%s takes %u iterations to reach L2-norm of residual %.5e\n",
                 GetSolType(params.type), numIter, norm2);
   }

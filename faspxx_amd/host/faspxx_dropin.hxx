@@ -18,6 +18,8 @@
 #pragma once
 
 #include <cstdint>
+#include <iostream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -55,7 +57,19 @@ enum FaspRetCode {
   ERROR_SOLVER_SOLSTAG = -33,
   ERROR_SOLVER_TOLSMALL = -34,
   ERROR_SOLVER_MAXIT = -39,
-  ERROR_DEVICE = -90  // HIP runtime failure (no reference counterpart)
+  ERROR_AMG_INTERP_TYPE = -40,
+  ERROR_AMG_SMOOTH_TYPE = -41,
+  ERROR_AMG_COARSE_TYPE = -42,
+  ERROR_AMG_COARSEING = -43,
+  ERROR_AMG_SETUP = -49,
+  ERROR_ILU_TYPE = -50,
+  ERROR_ILU_SETUP = -59,
+  ERROR_SWZ_TYPE = -60,
+  ERROR_SWZ_SETUP = -69,
+  ERROR_DEVICE = -90,  // HIP runtime failure (no reference counterpart)
+  ERROR_DSOLVER_SETUP = -91,
+  ERROR_DSOLVER_SOLVE = -92,
+  PRINT_HELP = 0
 };
 
 enum Output { PRINT_NONE = 0, PRINT_MIN = 2, PRINT_SOME = 4, PRINT_MORE = 6, PRINT_MAX = 8 };
@@ -90,6 +104,48 @@ struct SOLParams {
   double relTol = 1e-6;
   double absTol = 1e-8;
   Output verbose = PRINT_NONE;
+};
+
+// Parameters: program defaults < parameter file < command line
+// (src/Param.hxx:82-226; defined in faspxx_param.cpp).  marker: 0 optional,
+// 1 required (only shown in the help), 2 names the parameter file.
+class Parameters {
+ public:
+  Parameters(int argc, const char* argv[]) : argc(argc), argv(argv) {}
+  void AddParam(const std::string& name, const std::string& help, bool* ptr, int marker = 0);
+  void AddParam(const std::string& name, const std::string& help, USI* ptr, int marker = 0);
+  void AddParam(const std::string& name, const std::string& help, INT* ptr, int marker = 0);
+  void AddParam(const std::string& name, const std::string& help, double* ptr, int marker = 0);
+  void AddParam(const std::string& name, const std::string& help, std::string* ptr,
+                int marker = 0);
+  void AddParam(const std::string& name, const std::string& help, Output* ptr, int marker = 0);
+  void Parse();
+  void SetSOLParams(SOLParams& solParam);
+  void SetSOLParams(SOLParams& solParam, SOLParams& pcdParam);
+  void PrintUserParams(std::ostream& out) const;
+  void PrintFileParams(std::ostream& out) const;
+  void PrintCommandLineParams(std::ostream& out) const;
+  void Print(std::ostream& out = std::cout) const;
+  void PrintHelp(std::ostream& out = std::cout) const;
+
+ private:
+  enum Kind { KBool = 0, KInt = 1, KDouble = 2, KString = 3, KOutput = 4 };
+  struct Entry {
+    Kind kind;
+    std::string name, help;
+    void* ptr;
+    int marker;
+  };
+  int argc;
+  const char** argv;
+  std::vector<Entry> entries;
+  std::map<std::string, std::string> fromCommandLine, fromFile;
+  std::string original;
+  void Add(Kind kind, const std::string& name, const std::string& help, void* ptr, int marker);
+  void ReadCommandLine();
+  void ReadFile();
+  static std::string ValueText(const Entry& e);
+  static void Assign(Entry& e, const std::string& text);
 };
 
 namespace faspxx_amd {
