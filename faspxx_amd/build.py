@@ -93,7 +93,7 @@ def build_dropin() -> None:
     if p.returncode != 0:
         raise RuntimeError(f"drop-in library build failed:\n{p.stderr}")
     progs = [(ROOT / "tests" / "cpp" / f"{t}.cpp", ROOT / "tests" / "cpp" / t, "$ORIGIN/../../faspxx_amd")
-             for t in ("dropin_tests", "readdata_tests", "param_tests")]
+             for t in ("dropin_tests", "readdata_tests", "param_tests", "mg_tests")]
     progs += [(PKG / "examples" / f"{t}.cpp", PKG / "examples" / t, "$ORIGIN/..") for t in EXAMPLES]
     for src, out, rpath in progs:
         cmd = [cxx, *common, "-o", str(out), str(src), f"-L{PKG}", "-lfaspxx_dropin",

@@ -301,8 +301,12 @@ MAT::~MAT() { Release(); }
 
 MAT& MAT::operator=(const MAT& m) {
   if (this == &m) return *this;
-  if (!m.csr) {
-    Build(0, 0, 0, nullptr, nullptr, nullptr, nullptr);
+  if (!m.csr) {  // no entries: sizes only (an IdentityMatrix keeps its size)
+    Release();
+    nrow = m.nrow;
+    mcol = m.mcol;
+    nnz = 0;
+    has_values = false;
     return *this;
   }
   std::vector<USI> rp(m.nrow + 1), ci(m.nnz), dp(std::min(m.nrow, m.mcol));
@@ -349,6 +353,11 @@ void MAT::Apply(const VEC& v, VEC& w) const {
 void MAT::Residual(const VEC& b, const VEC& x, VEC& r) const {
   if (nrow == 0) return;
   check(fx_residual(Context(), csr, b.Device(), x.Device(), r.DeviceOut()), "MAT::Residual");
+}
+
+IdentityMatrix::IdentityMatrix(USI size) {
+  nrow = size;
+  mcol = size;
 }
 
 void MAT::TransInPlace() {

@@ -281,6 +281,15 @@ class MAT : public LOP {
   void Release();
 };
 
+// IdentityMatrix (src/MAT.hxx:215-232): a MAT of the given size with no
+// entries whose Apply copies (src/MAT.cxx:1088-1095).
+class IdentityMatrix : public MAT {
+ public:
+  IdentityMatrix() = default;
+  explicit IdentityMatrix(USI size);
+  void Apply(const VEC& b, VEC& x) const override { x = b; }
+};
+
 class SOL {
  protected:
   const LOP* A;

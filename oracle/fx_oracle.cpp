@@ -20,6 +20,8 @@
 //       iteration (reference reads an uninitialised alpha, src/CG.cxx:109).
 //   D4  An empty CSR row yields 0 in SpMV (reference reads the next row's
 //       first entry, src/MAT.cxx:415; CSRx forbids empty rows anyway).
+//   D5  MG::SetupLevel with level == numLevelsCoarse is rejected (reference:
+//       writes past the end of infoHL, src/MG.cxx:27-31).
 //
 // Pinning: see tests/test_oracle_*.py -- reference unit-test golden values,
 // the compiled reference (oracle/_ref) on the same inputs, closed forms on
@@ -990,6 +992,121 @@ int orc_solve(int kind, const orc_csr* A, const orc_pc* pc, const orc_params* pr
     case ORC_JACOBI: return jacobi(c, pc->dinv, b, x, res);
     default: res->retcode = RC_SOLVER_TYPE; return RC_SOLVER_TYPE;
   }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multigrid -- MG<MAT>, src/MG.cxx:16-211, as example/TestMG.cxx:81-99 sets it
+// up: every level's smoother and coarse solver is a Jacobi solver on the
+// fine matrix A (Jacobi::Setup(mat), src/Iter.cxx:29-53), and its transfer
+// operator serves as both restriction and prolongation through Apply.  The
+// cycle forms every level's residual with the fine A into the one fine-size
+// work vector r (src/MG.cxx:135,156), so the transfer operators must be
+// square (n x n); an IdentityMatrix transfer (src/MAT.cxx:1088-1095) copies.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+struct orc_mg_level {
+  const orc_csr* transfer;  // null: IdentityMatrix
+  orc_params smooth;        // pre- and post-smoother (one Jacobi object)
+  orc_params coarse;        // coarse solver, used on the last level
+};
+
+}  // extern "C"
+
+namespace {
+
+struct MgRun {
+  const orc_csr* A;
+  const double* dinv;
+  uint32_t nlev;
+  const orc_mg_level* lv;
+  const uint32_t* ncycles;
+  uint32_t n;
+  int level = -1;
+  Vec r;
+  std::vector<Vec> hb, hx;  // infoHL[l].b, infoHL[l].x
+
+  void jacobi_solve(const orc_params& p, const double* b, double* x) const {
+    Ctx c{A, nullptr, p, n, nullptr, 0};
+    orc_result rr{};
+    jacobi(c, dinv, b, x, &rr);
+  }
+  void apply(const orc_mg_level& L, const double* in, double* out) const {
+    if (L.transfer)
+      orc_spmv(L.transfer, in, out);
+    else
+      std::memcpy(out, in, sizeof(double) * n);
+  }
+  // MG::MGCycle, src/MG.cxx:117-172
+  void cycle(const double* b, double* x) {
+    if (nlev == 0) {  // "No multilevel info available": x = b
+      std::memcpy(x, b, sizeof(double) * n);
+      return;
+    }
+    ++level;
+    if (level == (int)nlev) return;
+    const orc_mg_level& L = lv[level];
+    double* bl = hb[level].data();
+    double* xl = hx[level].data();
+    jacobi_solve(L.smooth, b, x);
+    orc_residual(A, b, x, r.data());
+    apply(L, r.data(), bl);
+    std::fill(hx[level].begin(), hx[level].end(), 0.0);
+    if (level + 1 == (int)nlev)
+      jacobi_solve(L.coarse, bl, xl);
+    else
+      for (uint32_t i = 0; i < ncycles[level]; ++i) cycle(bl, xl);
+    apply(L, xl, r.data());
+    orc_axpy(n, 1.0, r.data(), x);
+    jacobi_solve(L.smooth, b, x);
+    --level;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// MG::Solve, src/MG.cxx:175-229.  dinv: the smoothers' diagInv (weight / diag
+// of A).  hist (optional) receives resAbs at each checked iteration.
+int orc_mg(const orc_csr* A, const double* dinv, uint32_t nlev, const orc_mg_level* lv,
+           const uint32_t* ncycles, const orc_params* prm, const double* b, double* x,
+           orc_result* res, double* hist, uint32_t hist_cap) {
+  const uint32_t n = A->nrow;
+  for (uint32_t l = 0; l < nlev; ++l) {
+    const orc_csr* T = lv[l].transfer;
+    if (T && (T->nrow != n || T->mcol != n)) return RC_INPUT_PAR;
+  }
+  MgRun run{A, dinv, nlev, lv, ncycles, n};
+  run.r.assign(n, 0.0);
+  run.hb.assign(nlev, Vec(n, 0.0));
+  run.hx.assign(nlev, Vec(n, 0.0));
+  const orc_params& P = *prm;
+  double resAbs = 1.0, denAbs = 1.0;
+  uint32_t numIter = 0;
+  res->retcode = RC_SUCCESS;
+  res->norm2 = 1e60;
+  res->normInf = 1e60;
+  while (numIter < P.maxIter) {
+    orc_residual(A, b, x, run.r.data());
+    if (numIter >= P.minIter) {
+      resAbs = orc_norm2(n, run.r.data());
+      if (numIter == P.minIter) denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+      if (resAbs / denAbs < P.relTol || resAbs < P.absTol) break;
+      if (hist && numIter < hist_cap) hist[numIter] = resAbs;
+    }
+    run.cycle(b, x);
+    ++numIter;
+  }
+  res->numIter = numIter;
+  if (!(numIter == P.minIter && numIter == P.maxIter)) {
+    orc_residual(A, b, x, run.r.data());
+    res->norm2 = orc_norm2(n, run.r.data());
+    res->normInf = orc_norminf(n, run.r.data());
+  }
+  return RC_SUCCESS;
 }
 
 }  // extern "C"

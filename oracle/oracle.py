@@ -43,6 +43,10 @@ class _Pc(C.Structure):
                 ("color", C.c_void_p), ("ncolors", C.c_uint32), ("weight", C.c_double)]
 
 
+class _MgLevel(C.Structure):
+    _fields_ = [("transfer", C.POINTER(_Csr)), ("smooth", _Params), ("coarse", _Params)]
+
+
 class _Result(C.Structure):
     _fields_ = [("retcode", C.c_int), ("numIter", C.c_uint32), ("norm2", C.c_double),
                 ("normInf", C.c_double)]
@@ -114,6 +118,8 @@ class Oracle:
             "orc_mcgs": (None, [cs, v, u, d, v, v, u, C.c_int]),
             "orc_solve": (C.c_int, [C.c_int, cs, C.POINTER(_Pc), C.POINTER(_Params), v, v,
                                     C.POINTER(_Result), v, u]),
+            "orc_mg": (C.c_int, [cs, v, u, C.POINTER(_MgLevel), v, C.POINTER(_Params), v, v,
+                                 C.POINTER(_Result), v, u]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -228,6 +234,40 @@ class Oracle:
         if hist is not None:
             out["hist"] = hist
         return out
+
+    def mg(self, M: Matrix, b, x0, levels, ncycles=1, weight=1.0, hist_cap=0, **prm):
+        """MG<MAT>::Solve with `levels` = [(transfer, smooth_prm, coarse_prm)]:
+        transfer a square Matrix or None (IdentityMatrix), the two dicts the
+        SOLParams of the level's Jacobi smoother and coarse solver (defaults
+        as SOLParams()).  ncycles: an int for every level or a list."""
+        nlev = len(levels)
+        keep = []
+        arr = (_MgLevel * max(nlev, 1))()
+        for i, (T, sp, cp) in enumerate(levels):
+            arr[i].transfer = C.pointer(T.c) if T is not None else None
+            arr[i].smooth = _params(**(sp or {}))
+            arr[i].coarse = _params(**(cp or {}))
+            keep.append(T)
+        nc = np.full(max(nlev, 1), ncycles, np.uint32) if np.isscalar(ncycles) else \
+            _u32(ncycles)
+        dinv = self.jacobi_dinv(M, weight)
+        b = _f64(b)
+        x = _f64(x0).copy()
+        res = _Result()
+        hist = np.zeros(hist_cap) if hist_cap else None
+        rc = self.L.orc_mg(C.byref(M.c), _p(dinv), nlev, arr, _p(nc), C.byref(_params(**prm)),
+                           _p(b), _p(x), C.byref(res), _p(hist), hist_cap)
+        out = {"retcode": rc, "numIter": res.numIter, "norm2": res.norm2,
+               "normInf": res.normInf, "x": x}
+        if hist is not None:
+            out["hist"] = hist
+        return out
+
+
+def _params(**prm) -> _Params:
+    P = dict(DEFAULTS)
+    P.update(prm)
+    return _Params(**P)
 
 
 class Reference:
