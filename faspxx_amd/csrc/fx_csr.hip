@@ -274,6 +274,30 @@ void csr_free(fx_csr A) {
 
 }  // namespace
 
+int csr_storage(fx_csr A) { return csr_alloc(A); }
+
+void csr_release(fx_csr A) { csr_free(A); }
+
+int scan_u32(hipStream_t st, const uint32_t* cnt, uint32_t m, uint32_t* out) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cnt, m, out);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+int csr_finish_device(fx_csr A) {
+  hipStream_t st = A->ctx->stream;
+  if (A->nrow) {
+    hipLaunchKernelGGL(k_form_diagptr, dim3((A->nrow + kThreads - 1) / kThreads), dim3(kThreads),
+                       0, st, A->rp, A->ci, A->dp, A->nrow);
+    FX_HIP(hipGetLastError());
+  }
+  std::vector<uint32_t> hrp((size_t)A->nrow + 1);
+  FX_HIP(hipMemcpyAsync(hrp.data(), A->rp, sizeof(uint32_t) * hrp.size(), hipMemcpyDeviceToHost,
+                        st));
+  FX_HIP(hipStreamSynchronize(st));
+  return csr_set_blocks(A, hrp.data());
+}
+
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype, hipEvent_t e0,
                       hipEvent_t e1) {
   if (A->nrow == 0) return 0;

@@ -397,6 +397,16 @@ __global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict
   f.fin(acc);
 }
 
+// Building a matrix on the device (fx_csr.hip): allocate rp / ci / dp / va
+// for A's nrow and nnz; free a matrix; exclusive scan of cnt[0..m) into
+// out[0..m] (out[m] = total; one workgroup); and, once rp / ci / va are
+// filled, form diagPtr (first match, as FormDiagPtr) and the row-block
+// tables.
+int csr_storage(fx_csr A);
+void csr_release(fx_csr A);
+int scan_u32(hipStream_t st, const uint32_t* cnt, uint32_t m, uint32_t* out);
+int csr_finish_device(fx_csr A);
+
 template <int K, unsigned MaxMask, class Fin>
 inline int launch_finish(hipStream_t st, const double* partials, unsigned G, Fin f) {
   hipLaunchKernelGGL((k_finish<K, MaxMask, Fin>), dim3(1), dim3(kFinThreads), 0, st, partials, G,

@@ -123,6 +123,11 @@ _SIGS = {
     "fx_csr_get_diag": (C.c_int, [_P, _P, _P]),
     "fx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "fx_csr_transpose_update": (C.c_int, [_P, _P, _P]),
+    "fx_csr_lower_tri": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "fx_csr_upper_tri": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "fx_csr_add": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, C.POINTER(_P)]),
+    "fx_csr_mult": (C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    "fx_csr_shift_diag": (C.c_int, [_P, _P, C.c_double]),
     "fx_csr_pattern_grad": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_pc_create_identity": (C.c_int, [_P, C.POINTER(_P)]),
     "fx_pc_create_mcgs": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
@@ -423,6 +428,36 @@ class Csr:
         h = C.c_void_p()
         _chk(load().fx_csr_transpose(self.ctx.h, self.h, C.byref(h)))
         return Csr._from_handle(self.ctx, h)
+
+    def lower_tri(self) -> "Csr":
+        """MAT::GetLowerTri (src/MAT.cxx:290-316)."""
+        h = C.c_void_p()
+        _chk(load().fx_csr_lower_tri(self.ctx.h, self.h, C.byref(h)))
+        return Csr._from_handle(self.ctx, h)
+
+    def upper_tri(self) -> "Csr":
+        """MAT::GetUpperTri (src/MAT.cxx:319-345)."""
+        h = C.c_void_p()
+        _chk(load().fx_csr_upper_tri(self.ctx.h, self.h, C.byref(h)))
+        return Csr._from_handle(self.ctx, h)
+
+    @staticmethod
+    def add(a: float, A: "Csr", b: float, B: "Csr") -> "Csr":
+        """a A + b B (MAT::Add, src/MAT.cxx:652-719)."""
+        h = C.c_void_p()
+        _chk(load().fx_csr_add(A.ctx.h, a, A.h, b, B.h, C.byref(h)))
+        return Csr._from_handle(A.ctx, h)
+
+    @staticmethod
+    def mult(L: "Csr", R: "Csr") -> "Csr":
+        """L R (MAT::Mult, src/MAT.cxx:722-800)."""
+        h = C.c_void_p()
+        _chk(load().fx_csr_mult(L.ctx.h, L.h, R.h, C.byref(h)))
+        return Csr._from_handle(L.ctx, h)
+
+    def shift_diag(self, a: float):
+        """values[diagPtr[j]] += a (MAT::Shift, src/MAT.cxx:359-364)."""
+        _chk(load().fx_csr_shift_diag(self.ctx.h, self.h, a))
 
     def transpose_update(self, At: "Csr"):
         """Refresh At's values from this matrix (At = self.transpose())."""

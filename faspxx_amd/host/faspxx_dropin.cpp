@@ -355,6 +355,106 @@ void MAT::Residual(const VEC& b, const VEC& x, VEC& r) const {
   check(fx_residual(Context(), csr, b.Device(), x.Device(), r.DeviceOut()), "MAT::Residual");
 }
 
+void MAT::Adopt(fx_csr h, bool values) {
+  Release();
+  uint32_t nr = 0, mc = 0, nz = 0;
+  check(fx_csr_info(h, &nr, &mc, &nz), "fx_csr_info");
+  if (nz == 0) {  // MAT::Empty
+    fx_csr_destroy(h);
+    nrow = mcol = nnz = 0;
+    has_values = false;
+    return;
+  }
+  csr = h;
+  nrow = nr;
+  mcol = mc;
+  nnz = nz;
+  has_values = values;
+}
+
+void MAT::GetDiagInv(MAT& m) const {  // src/MAT.cxx:270-286: 1 / values[diagPtr[j]]
+  VEC d;
+  GetDiag(d);
+  d.Reciprocal();
+  m = MAT(d);
+}
+
+void MAT::GetLowerTri(MAT& lTri) const {
+  if (!csr) return lTri.Empty();
+  fx_csr h = nullptr;
+  check(fx_csr_lower_tri(Context(), csr, &h), "fx_csr_lower_tri");
+  lTri.Adopt(h, has_values);
+}
+
+void MAT::GetUpperTri(MAT& uTri) const {
+  if (!csr) return uTri.Empty();
+  fx_csr h = nullptr;
+  check(fx_csr_upper_tri(Context(), csr, &h), "fx_csr_upper_tri");
+  uTri.Adopt(h, has_values);
+}
+
+void MAT::Shift(const DBL a) {
+  if (!csr || !has_values) return;
+  check(fx_csr_shift_diag(Context(), csr, a), "fx_csr_shift_diag");
+}
+
+void MAT::Zero() {  // src/MAT.cxx:367-371
+  if (!csr || !has_values) return;
+  fx_vec vals = nullptr;
+  check(fx_vec_wrap(Context(), nnz, FX_F64, fx_csr_values_dev(csr), &vals), "fx_vec_wrap");
+  check(fx_vec_set(vals, 0.0), "fx_vec_set");
+  fx_vec_destroy(vals);
+  check(fx_csr_set_values_dev(csr, fx_csr_values_dev(csr)), "fx_csr_set_values_dev");
+}
+
+double* MAT::GetValues() const {
+  double* v = new double[nnz ? nnz : 1];
+  if (csr && has_values) check(fx_csr_download(csr, nullptr, nullptr, v, nullptr), "fx_csr_download");
+  return v;
+}
+
+int* MAT::GetRowPtr() const {
+  int* r = new int[(size_t)nrow + 1]();
+  if (csr) check(fx_csr_download(csr, (uint32_t*)r, nullptr, nullptr, nullptr), "fx_csr_download");
+  return r;
+}
+
+int* MAT::GetColInd() const {
+  int* c = new int[nnz ? nnz : 1];
+  if (csr) check(fx_csr_download(csr, nullptr, (uint32_t*)c, nullptr, nullptr), "fx_csr_download");
+  return c;
+}
+
+void MAT::Add(const DBL a, const MAT& mat1, const DBL b, const MAT& mat2) {
+  if (mat1.nnz == 0 || mat2.nnz == 0) return;
+  fx_csr h = nullptr;
+  check(fx_csr_add(Context(), a, mat1.csr, b, mat2.csr, &h), "fx_csr_add");
+  Adopt(h, true);
+}
+
+void MAT::Mult(const MAT& matl, const MAT& matr) {
+  if (!matl.csr || !matr.csr) return Empty();
+  fx_csr h = nullptr;
+  check(fx_csr_mult(Context(), matl.csr, matr.csr, &h), "fx_csr_mult");
+  Adopt(h, matl.has_values && matr.has_values);
+}
+
+void MAT::MultLeft(const MAT& mat) {
+  MAT tmp(*this);
+  Mult(tmp, mat);
+}
+
+void MAT::MultRight(const MAT& mat) {
+  MAT tmp(*this);
+  Mult(mat, tmp);
+}
+
+void MAT::Empty() {
+  Release();
+  nrow = mcol = nnz = 0;
+  has_values = false;
+}
+
 IdentityMatrix::IdentityMatrix(USI size) {
   nrow = size;
   mcol = size;

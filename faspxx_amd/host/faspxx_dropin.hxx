@@ -269,6 +269,25 @@ class MAT : public LOP {
   void TransInPlace();
   DBL GetValue(const USI& row, const USI& col) const;
 
+  // Setup-time algebra (src/MAT.cxx:270-372, 621-800, 917-935), on the device.
+  void GetDiagInv(MAT& m) const;
+  void GetLowerTri(MAT& lTri) const;
+  void GetUpperTri(MAT& uTri) const;
+  void Shift(const DBL a);
+  void Zero();
+  // new[]-allocated host copies, as the reference returns them
+  double* GetValues() const;
+  int* GetRowPtr() const;
+  int* GetColInd() const;
+  // *this = a mat1 + b mat2; left unchanged when either has no entries, as
+  // the reference leaves it (src/MAT.cxx:659-669)
+  void Add(const DBL a, const MAT& mat1, const DBL b, const MAT& mat2);
+  void Mult(const MAT& matl, const MAT& matr);
+  void MultLeft(const MAT& mat);   // *this = *this mat
+  void MultRight(const MAT& mat);  // *this = mat *this
+  void FormDiagPtr() {}            // formed on the device whenever the structure changes
+  void Empty();
+
   // Device handle (no reference counterpart).
   fx_csr Device() const { return csr; }
 
@@ -279,6 +298,7 @@ class MAT : public LOP {
   void Build(USI nrow, USI mcol, USI nnz, const DBL* values, const USI* colInd,
              const USI* rowPtr, const USI* diagPtr);
   void Release();
+  void Adopt(fx_csr h, bool values);  // take over a handle made by the C ABI
 };
 
 // IdentityMatrix (src/MAT.hxx:215-232): a MAT of the given size with no

@@ -202,6 +202,30 @@ int fx_csr_get_diag(fx_ctx ctx, fx_csr A, fx_vec v);
  * ascending in each row; diagPtr formed as FormDiagPtr does). */
 int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At);
 
+/* ---- setup-time sparse algebra: each result is a new handle -----------
+ * Lower / upper triangular part by position: row i keeps its entries up to
+ * and including its diagonal entry, or from it to the row's end; every row
+ * needs a diagonal entry (FX_ERROR_MAT_DATA otherwise).
+ *   MAT::GetLowerTri / GetUpperTri, src/MAT.cxx:290-345 */
+int fx_csr_lower_tri(fx_ctx ctx, fx_csr A, fx_csr* L);
+int fx_csr_upper_tri(fx_ctx ctx, fx_csr A, fx_csr* U);
+/* *C = a A + b B: per row the union of the patterns, columns ascending; an
+ * entry in both is fma(b, B_ij, a A_ij).  Both need values and one shape.
+ * With one operand empty the result is the other, scaled (the reference
+ * then leaves its target unchanged; the drop-in MAT::Add keeps that).
+ *   MAT::Add, src/MAT.cxx:652-719 */
+int fx_csr_add(fx_ctx ctx, double a, fx_csr A, double b, fx_csr B, fx_csr* C);
+/* *C = L R: row i holds the distinct columns reached through row i of L in
+ * order of first appearance (not sorted), values the fma chain of the
+ * products in that order.  Structure-only operands give a structure-only
+ * product; more than 2^31 candidate products is FX_ERROR_ALLOC_MEM.
+ *   MAT::Mult, src/MAT.cxx:722-800 */
+int fx_csr_mult(fx_ctx ctx, fx_csr L, fx_csr R, fx_csr* C);
+/* values[diagPtr[j]] += a in place, for every row whose diagPtr names its
+ * diagonal entry (the reference also adds to the entry a stale diagPtr names
+ * in a row without one).   MAT::Shift, src/MAT.cxx:359-364 */
+int fx_csr_shift_diag(fx_ctx ctx, fx_csr A, double a);
+
 /* Refresh the values of At (made by fx_csr_transpose from A) from A's
  * current values, without recomputing the structure.  No reference
  * counterpart: the adjoint solve's A^T after A's values change. */

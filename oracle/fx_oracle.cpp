@@ -22,6 +22,10 @@
 //       first entry, src/MAT.cxx:415; CSRx forbids empty rows anyway).
 //   D5  MG::SetupLevel with level == numLevelsCoarse is rejected (reference:
 //       writes past the end of infoHL, src/MG.cxx:27-31).
+//   D6  Row sorting after MAT::Add uses a signed insertion index (reference
+//       SortCSRRow: `USI l; l >= begin` wraps when an entry moves to the front
+//       of row 0, src/MATUtil.cxx:454); MAT::Add's merge window, which may
+//       read one slot past its buffer, gets that slot allocated.
 //
 // Pinning: see tests/test_oracle_*.py -- reference unit-test golden values,
 // the compiled reference (oracle/_ref) on the same inputs, closed forms on
@@ -29,6 +33,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1107,6 +1112,166 @@ int orc_mg(const orc_csr* A, const double* dinv, uint32_t nlev, const orc_mg_lev
     res->normInf = orc_norminf(n, run.r.data());
   }
   return RC_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Setup-time sparse algebra -- src/MAT.cxx:290-345 (GetLowerTri, GetUpperTri),
+// :652-719 (Add), :722-800 (Mult).  Results are returned in malloc'd arrays
+// (orc_mat_free releases them).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+struct orc_mat_out {
+  uint32_t nrow, mcol, nnz;
+  uint32_t* rp;
+  uint32_t* ci;
+  double* va;  // null for a structure-only result
+  uint32_t* dp;
+};
+
+void orc_mat_free(orc_mat_out* m) {
+  std::free(m->rp);
+  std::free(m->ci);
+  std::free(m->va);
+  std::free(m->dp);
+  m->rp = m->ci = m->dp = nullptr;
+  m->va = nullptr;
+}
+
+}  // extern "C"
+
+namespace {
+
+void emit(uint32_t nrow, uint32_t mcol, const std::vector<uint32_t>& rp,
+          const std::vector<uint32_t>& ci, const std::vector<double>* va, orc_mat_out* out) {
+  const uint32_t nnz = rp[nrow];
+  out->nrow = nrow;
+  out->mcol = mcol;
+  out->nnz = nnz;
+  out->rp = (uint32_t*)std::malloc(sizeof(uint32_t) * (nrow + 1));
+  out->ci = (uint32_t*)std::malloc(sizeof(uint32_t) * (nnz ? nnz : 1));
+  out->va = va ? (double*)std::malloc(sizeof(double) * (nnz ? nnz : 1)) : nullptr;
+  out->dp = (uint32_t*)std::calloc(nrow ? nrow : 1, sizeof(uint32_t));
+  std::memcpy(out->rp, rp.data(), sizeof(uint32_t) * (nrow + 1));
+  std::memcpy(out->ci, ci.data(), sizeof(uint32_t) * nnz);
+  if (va) std::memcpy(out->va, va->data(), sizeof(double) * nnz);
+  orc_form_diagptr(nrow, out->rp, out->ci, out->dp);  // FormDiagPtr, src/MAT.cxx:917-928
+}
+
+}  // namespace
+
+extern "C" {
+
+// MAT::GetLowerTri / GetUpperTri: row j keeps [rowPtr[j], diagPtr[j]] or
+// [diagPtr[j], rowPtr[j+1]) by position.
+void orc_mat_tri(const orc_csr* A, int upper, orc_mat_out* out) {
+  const uint32_t n = A->nrow;
+  std::vector<uint32_t> rp(n + 1, 0), ci;
+  std::vector<double> va;
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t s = upper ? A->dp[j] : A->rp[j];
+    const uint32_t e = upper ? A->rp[j + 1] : A->dp[j] + 1;
+    for (uint32_t k = s; k < e; ++k) {
+      ci.push_back(A->ci[k]);
+      if (A->va) va.push_back(A->va[k]);
+    }
+    rp[j + 1] = (uint32_t)ci.size();
+  }
+  emit(n, A->mcol, rp, ci, A->va ? &va : nullptr, out);
+}
+
+// MAT::Add: returns 1 (target unchanged) when either operand has no entries,
+// as the reference returns before assigning its result (src/MAT.cxx:659-669).
+int orc_mat_add(double a, const orc_csr* A, double b, const orc_csr* B, orc_mat_out* out) {
+  if (A->nnz == 0 || B->nnz == 0) return 1;
+  const uint32_t n = A->nrow, cap = A->nnz + B->nnz;
+  std::vector<uint32_t> rp(n + 1, 0), ci(cap + 1, 0xffffffffu);
+  std::vector<double> va(cap + 1, 0.0);
+  uint32_t count = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t countrow = 0;
+    for (uint32_t j = A->rp[i]; j < A->rp[i + 1]; ++j) {
+      va[count] = a * A->va[j];
+      ci[count] = A->ci[j];
+      ++rp[i + 1];
+      ++count;
+      ++countrow;
+    }
+    for (uint32_t k = B->rp[i]; k < B->rp[i + 1]; ++k) {
+      bool added = false;
+      // the reference's window: this row's entries from A plus one slot
+      for (uint32_t l = rp[i]; l < rp[i] + countrow + 1; ++l) {
+        if (B->ci[k] == ci[l]) {
+          va[l] = std::fma(b, B->va[k], va[l]);
+          added = true;
+          break;
+        }
+      }
+      if (!added) {
+        va[count] = b * B->va[k];
+        ci[count] = B->ci[k];
+        ++rp[i + 1];
+        ++count;
+      }
+    }
+    rp[i + 1] += rp[i];
+  }
+  // SortCSRRow (src/MATUtil.cxx:439-465): insertion sort of each row (D6)
+  for (uint32_t j = 0; j < n; ++j) {
+    for (int64_t k = (int64_t)rp[j] + 1; k < (int64_t)rp[j + 1]; ++k) {
+      const uint32_t c = ci[k];
+      const double v = va[k];
+      int64_t l = k - 1;
+      for (; l >= (int64_t)rp[j] && c < ci[l]; --l) {
+        ci[l + 1] = ci[l];
+        va[l + 1] = va[l];
+      }
+      ci[l + 1] = c;
+      va[l + 1] = v;
+    }
+  }
+  emit(n, A->mcol, rp, ci, &va, out);
+  return 0;
+}
+
+// MAT::Mult: columns in order of first appearance, values summed over the
+// (k, l) products in order (structure-only operands: no values).
+void orc_mat_mult(const orc_csr* L, const orc_csr* R, orc_mat_out* out) {
+  const uint32_t n = L->nrow, m = R->mcol;
+  std::vector<uint32_t> rp(n + 1, 0), ci, seen;
+  seen.reserve(m);
+  for (uint32_t i = 0; i < n; ++i) {
+    seen.clear();
+    for (uint32_t k = L->rp[i]; k < L->rp[i + 1]; ++k) {
+      const uint32_t r = L->ci[k];
+      for (uint32_t j = R->rp[r]; j < R->rp[r + 1]; ++j) {
+        const uint32_t c = R->ci[j];
+        uint32_t l = 0;
+        while (l < seen.size() && seen[l] != c) ++l;
+        if (l == seen.size()) seen.push_back(c);
+      }
+    }
+    ci.insert(ci.end(), seen.begin(), seen.end());
+    rp[i + 1] = (uint32_t)ci.size();
+  }
+  const bool values = L->va && R->va;
+  std::vector<double> va(values ? ci.size() : 0);
+  if (values) {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) {
+        double v = 0.0;
+        for (uint32_t k = L->rp[i]; k < L->rp[i + 1]; ++k) {
+          const uint32_t r = L->ci[k];
+          for (uint32_t l = R->rp[r]; l < R->rp[r + 1]; ++l)
+            if (R->ci[l] == ci[j]) v = std::fma(L->va[k], R->va[l], v);
+        }
+        va[j] = v;
+      }
+    }
+  }
+  emit(n, m, rp, ci, values ? &va : nullptr, out);
 }
 
 }  // extern "C"

@@ -47,6 +47,30 @@ class _MgLevel(C.Structure):
     _fields_ = [("transfer", C.POINTER(_Csr)), ("smooth", _Params), ("coarse", _Params)]
 
 
+class _MatOut(C.Structure):
+    _fields_ = [("nrow", C.c_uint32), ("mcol", C.c_uint32), ("nnz", C.c_uint32),
+                ("rp", C.POINTER(C.c_uint32)), ("ci", C.POINTER(C.c_uint32)),
+                ("va", C.POINTER(C.c_double)), ("dp", C.POINTER(C.c_uint32))]
+
+
+class _RefMatOut(C.Structure):
+    _fields_ = [("nrow", C.c_uint32), ("mcol", C.c_uint32), ("nnz", C.c_uint32),
+                ("rp", C.POINTER(C.c_uint32)), ("ci", C.POINTER(C.c_uint32)),
+                ("va", C.POINTER(C.c_double))]
+
+
+def _take(o, with_dp):
+    """Copy a malloc'd result struct into numpy arrays (rp, ci, va | None, dp | None)."""
+    n, z = o.nrow, o.nnz
+    rp = np.ctypeslib.as_array(o.rp, (n + 1,)).copy()
+    ci = np.ctypeslib.as_array(o.ci, (z,)).copy() if z else np.zeros(0, np.uint32)
+    va = (np.ctypeslib.as_array(o.va, (z,)).copy() if z else np.zeros(0)) if o.va else None
+    dp = None
+    if with_dp:
+        dp = np.ctypeslib.as_array(o.dp, (n,)).copy() if n else np.zeros(0, np.uint32)
+    return rp, ci, va, dp
+
+
 class _Result(C.Structure):
     _fields_ = [("retcode", C.c_int), ("numIter", C.c_uint32), ("norm2", C.c_double),
                 ("normInf", C.c_double)]
@@ -120,6 +144,10 @@ class Oracle:
                                     C.POINTER(_Result), v, u]),
             "orc_mg": (C.c_int, [cs, v, u, C.POINTER(_MgLevel), v, C.POINTER(_Params), v, v,
                                  C.POINTER(_Result), v, u]),
+            "orc_mat_free": (None, [C.POINTER(_MatOut)]),
+            "orc_mat_tri": (None, [cs, C.c_int, C.POINTER(_MatOut)]),
+            "orc_mat_add": (C.c_int, [d, cs, d, cs, C.POINTER(_MatOut)]),
+            "orc_mat_mult": (None, [cs, cs, C.POINTER(_MatOut)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -235,6 +263,31 @@ class Oracle:
             out["hist"] = hist
         return out
 
+    # setup-time sparse algebra: each returns (nrow, mcol, rp, ci, va | None, dp)
+    def _mat(self, fn, *args):
+        o = _MatOut()
+        rc = fn(*args, C.byref(o))
+        if rc == 1:
+            return None
+        try:
+            return (o.nrow, o.mcol) + _take(o, True)
+        finally:
+            self.L.orc_mat_free(C.byref(o))
+
+    def lower_tri(self, M: Matrix):
+        return self._mat(self.L.orc_mat_tri, C.byref(M.c), 0)
+
+    def upper_tri(self, M: Matrix):
+        return self._mat(self.L.orc_mat_tri, C.byref(M.c), 1)
+
+    def mat_add(self, a, A: Matrix, b, B: Matrix):
+        """a A + b B as MAT::Add; None when the reference leaves its target
+        unchanged (an operand without entries)."""
+        return self._mat(self.L.orc_mat_add, a, C.byref(A.c), b, C.byref(B.c))
+
+    def mat_mult(self, L: Matrix, R: Matrix):
+        return self._mat(self.L.orc_mat_mult, C.byref(L.c), C.byref(R.c))
+
     def mg(self, M: Matrix, b, x0, levels, ncycles=1, weight=1.0, hist_cap=0, **prm):
         """MG<MAT>::Solve with `levels` = [(transfer, smooth_prm, coarse_prm)]:
         transfer a square Matrix or None (IdentityMatrix), the two dicts the
@@ -289,6 +342,11 @@ class Reference:
             "ref_axpby": (None, [u, d, d, v, v]), "ref_waxpby": (None, [u, d, v, d, v, v]),
             "ref_solve": (C.c_int, [C.c_int, u, u, v, v, v, v, C.c_int, u, d,
                                     C.POINTER(_RefParams), v, v, C.POINTER(_RefResult)]),
+            "ref_mat_free": (None, [C.POINTER(_RefMatOut)]),
+            "ref_mat_tri": (None, [u, u, u, v, v, v, v, C.c_int, C.POINTER(_RefMatOut)]),
+            "ref_mat_add": (C.c_int, [d, u, u, u, v, v, v, d, u, v, v, v,
+                                      C.POINTER(_RefMatOut)]),
+            "ref_mat_mult": (None, [u, u, u, v, v, v, u, u, v, v, v, C.POINTER(_RefMatOut)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -298,6 +356,28 @@ class Reference:
     @staticmethod
     def available() -> bool:
         return REF_SO.exists()
+
+    def _mat(self, fn, *args):
+        o = _RefMatOut()
+        rc = fn(*args, C.byref(o))
+        if rc == 1:
+            return None
+        try:
+            return (o.nrow, o.mcol) + _take(o, False)[:3]
+        finally:
+            self.L.ref_mat_free(C.byref(o))
+
+    def tri(self, M: Matrix, upper):
+        return self._mat(self.L.ref_mat_tri, M.nrow, M.mcol, M.nnz, _p(M.rp), _p(M.ci), _p(M.va),
+                         _p(M.dp), 1 if upper else 0)
+
+    def mat_add(self, a, A: Matrix, b, B: Matrix):
+        return self._mat(self.L.ref_mat_add, a, A.nrow, A.mcol, A.nnz, _p(A.rp), _p(A.ci),
+                         _p(A.va), b, B.nnz, _p(B.rp), _p(B.ci), _p(B.va))
+
+    def mat_mult(self, L: Matrix, R: Matrix):
+        return self._mat(self.L.ref_mat_mult, L.nrow, L.mcol, L.nnz, _p(L.rp), _p(L.ci),
+                         _p(L.va), R.mcol, R.nnz, _p(R.rp), _p(R.ci), _p(R.va))
 
     def _m(self, M: Matrix):
         return (M.nrow, M.mcol, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), _p(M.dp))

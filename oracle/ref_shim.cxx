@@ -12,6 +12,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -208,6 +209,89 @@ int ref_solve(int kind, uint32_t nrow, uint32_t nnz, const uint32_t* rp, const u
   res->retcode = rc;
   copy_out(xx, x);
   return rc;
+}
+
+}  // extern "C"
+
+// ---- setup-time sparse algebra: MAT::GetLowerTri / GetUpperTri / Add / Mult.
+// The result's arrays are malloc'd into *out (ref_mat_free releases them).
+extern "C" {
+
+struct ref_mat_out {
+  uint32_t nrow, mcol, nnz;
+  uint32_t* rp;
+  uint32_t* ci;
+  double* va;
+};
+
+void ref_mat_free(ref_mat_out* m) {
+  std::free(m->rp);
+  std::free(m->ci);
+  std::free(m->va);
+  m->rp = m->ci = nullptr;
+  m->va = nullptr;
+}
+
+}  // extern "C"
+
+namespace {
+
+void mat_out(const MAT& M, ref_mat_out* out) {
+  out->nrow = M.GetRowSize();
+  out->mcol = M.GetColSize();
+  out->nnz = M.GetNNZ();
+  const uint32_t n1 = out->nrow + 1, nz = out->nnz ? out->nnz : 1;
+  out->rp = (uint32_t*)std::malloc(sizeof(uint32_t) * n1);
+  out->ci = (uint32_t*)std::malloc(sizeof(uint32_t) * nz);
+  out->va = (double*)std::malloc(sizeof(double) * nz);
+  int* r = M.GetRowPtr();
+  int* c = M.GetColInd();
+  double* v = M.GetValues();
+  std::memcpy(out->rp, r, sizeof(uint32_t) * n1);
+  std::memcpy(out->ci, c, sizeof(uint32_t) * out->nnz);
+  std::memcpy(out->va, v, sizeof(double) * out->nnz);
+  delete[] r;
+  delete[] c;
+  delete[] v;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_mat_tri(uint32_t nrow, uint32_t mcol, uint32_t nnz, const uint32_t* rp,
+                 const uint32_t* ci, const double* va, const uint32_t* dp, int upper,
+                 ref_mat_out* out) {
+  MAT A = make_mat(nrow, mcol, nnz, rp, ci, va, dp), T;
+  if (upper)
+    A.GetUpperTri(T);
+  else
+    A.GetLowerTri(T);
+  mat_out(T, out);
+}
+
+// Returns 1 when the reference left the target unchanged (an operand empty).
+int ref_mat_add(double a, uint32_t nrow, uint32_t mcol, uint32_t annz, const uint32_t* arp,
+                const uint32_t* aci, const double* ava, double b, uint32_t bnnz,
+                const uint32_t* brp, const uint32_t* bci, const double* bva, ref_mat_out* out) {
+  MAT A = make_mat(nrow, mcol, annz, arp, aci, ava, nullptr);
+  MAT B = make_mat(nrow, mcol, bnnz, brp, bci, bva, nullptr);
+  MAT C;
+  C.Add(a, A, b, B);
+  if (C.GetNNZ() == 0) return 1;
+  mat_out(C, out);
+  return 0;
+}
+
+void ref_mat_mult(uint32_t lrow, uint32_t lcol, uint32_t lnnz, const uint32_t* lrp,
+                  const uint32_t* lci, const double* lva, uint32_t rcol, uint32_t rnnz,
+                  const uint32_t* rrp, const uint32_t* rci, const double* rva,
+                  ref_mat_out* out) {
+  MAT L = make_mat(lrow, lcol, lnnz, lrp, lci, lva, nullptr);
+  MAT R = make_mat(lcol, rcol, rnnz, rrp, rci, rva, nullptr);
+  MAT C;
+  C.Mult(L, R);
+  mat_out(C, out);
 }
 
 }  // extern "C"

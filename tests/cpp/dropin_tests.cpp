@@ -123,6 +123,80 @@ static void test_mat() {
   std::printf("ok mat\n");
 }
 
+// Setup-time algebra on the UnitTestsMAT.cxx:42-46 matrix: Zero (its
+// :179-187 section) and the operations it has no section for, against the
+// dense 4 x 4 arithmetic (small integers: exact).
+static void test_mat_algebra() {
+  const std::vector<DBL> values1 = {0, 1, 2, 3, 4, 5, 2, 1, 2, 1};
+  const std::vector<USI> colInd1 = {0, 2, 0, 1, 3, 0, 2, 3, 2, 3};
+  const std::vector<USI> rowPtr1 = {0, 2, 5, 8, 10};
+  const std::vector<USI> diagPtr1 = {0, 3, 6, 9};
+  const MAT mat1(4, 4, 10, values1, colInd1, rowPtr1, diagPtr1);
+  double A[4][4];
+  for (USI i = 0; i < 4; i++)
+    for (USI j = 0; j < 4; j++) A[i][j] = mat1.GetValue(i, j);
+  {
+    MAT m(mat1);
+    m.Zero();
+    for (USI i = 0; i < 4; i++)
+      for (USI j = 0; j < 4; j++) REQUIRE(m.GetValue(i, j) == 0);
+  }
+  MAT lo, up;
+  mat1.GetLowerTri(lo);
+  mat1.GetUpperTri(up);
+  REQUIRE(lo.GetNNZ() == 7 && up.GetNNZ() == 7);  // (0,0) is stored: both keep it
+  for (USI i = 0; i < 4; i++)
+    for (USI j = 0; j < 4; j++) {
+      REQUIRE(lo.GetValue(i, j) == (j <= i ? A[i][j] : 0.0));
+      REQUIRE(up.GetValue(i, j) == (j >= i ? A[i][j] : 0.0));
+    }
+  {
+    MAT m(mat1);
+    m.Shift(1.5);
+    for (USI i = 0; i < 4; i++)
+      for (USI j = 0; j < 4; j++) REQUIRE(m.GetValue(i, j) == A[i][j] + (i == j ? 1.5 : 0.0));
+  }
+  {
+    MAT s;
+    s.Add(2.0, up, -1.0, lo);  // union of the patterns: mat1's; the diagonal 2d - d
+    REQUIRE(s.GetNNZ() == 10);
+    for (USI i = 0; i < 4; i++)
+      for (USI j = 0; j < 4; j++)
+        REQUIRE(s.GetValue(i, j) == (j > i ? 2 * A[i][j] : j < i ? -A[i][j] : A[i][j]));
+    MAT e, keep(mat1);
+    keep.Add(1.0, e, 1.0, mat1);  // an empty operand leaves the target unchanged
+    REQUIRE(keep.GetNNZ() == 10 && keep.GetValue(1, 3) == 4.0);
+  }
+  {
+    MAT p;
+    p.Mult(mat1, mat1);
+    for (USI i = 0; i < 4; i++)
+      for (USI j = 0; j < 4; j++) {
+        double want = 0;
+        for (USI k = 0; k < 4; k++) want += A[i][k] * A[k][j];
+        REQUIRE(p.GetValue(i, j) == want);
+      }
+    MAT q(mat1);
+    q.MultLeft(mat1);
+    for (USI i = 0; i < 4; i++)
+      for (USI j = 0; j < 4; j++) REQUIRE(q.GetValue(i, j) == p.GetValue(i, j));
+    int* rp = p.GetRowPtr();
+    REQUIRE(rp[0] == 0 && (USI)rp[4] == p.GetNNZ());
+    delete[] rp;
+  }
+  {
+    MAT dinv;
+    mat1.GetDiagInv(dinv);
+    REQUIRE(dinv.GetNNZ() == 4 && dinv.GetValue(1, 1) == 1.0 / 3.0 && dinv.GetValue(3, 3) == 1.0);
+  }
+  {
+    MAT m(mat1);
+    m.Empty();
+    REQUIRE(m.GetRowSize() == 0 && m.GetNNZ() == 0);
+  }
+  std::printf("ok mat algebra\n");
+}
+
 static void test_jacobi() {
   // UnitTestsJacobi.cxx:85-157 (3x3, Octave solution, tol 1e-5)
   std::vector<DBL> value = {4, 2, 1, 3, 8, 2, 2, 1, 9};
@@ -262,6 +336,7 @@ int main(int argc, char** argv) {
   if (argc > 1) test_readmat(argv[1]);
   test_vec();
   test_mat();
+  test_mat_algebra();
   test_jacobi();
   test_solvers();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
