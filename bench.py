@@ -139,6 +139,34 @@ def leg_powerlaw(fx, P, ctx, flush, sink, warmup, reps):
     return out
 
 
+def leg_setup_algebra(fx, ctx, A, A64):
+    """Setup-time sparse algebra on the 3D Poisson 128^3 matrix: A A
+    (MAT::Mult), A + 0.5 A (MAT::Add), the lower triangular part; wall time
+    of the call (each ends with the result's row-block table on the host),
+    best of 2."""
+    rp = A64.row_ptr.astype(np.int64)
+    rowlen = np.diff(rp)
+    cand = int(np.sum(rowlen[A64.col_ind.astype(np.int64)]))  # candidate products of A A
+    out = {"workload": "poisson3d_128 (n = %d, nnz = %d)" % (A64.nrow, A64.nnz)}
+    for name, fn in (("mult_AA", lambda: fx.Csr.mult(A, A)),
+                     ("add_A_halfA", lambda: fx.Csr.add(1.0, A, 0.5, A)),
+                     ("lower_tri", lambda: A.lower_tri())):
+        ms, res = [], None
+        for _ in range(2):
+            ctx.sync()
+            t = time.perf_counter()
+            res = fn()
+            ctx.sync()
+            ms.append((time.perf_counter() - t) * 1e3)
+        out[name] = {"ms": round(min(ms), 2), "nnz_out": res.nnz}
+        res.close()
+    out["mult_AA"]["candidate_products"] = cand
+    out["mult_AA"]["Mproducts_per_s"] = round(cand / out["mult_AA"]["ms"] / 1e3, 1)
+    log(f"setup algebra: A A {out['mult_AA']['ms']} ms ({out['mult_AA']['nnz_out']} nnz), "
+        f"A + 0.5 A {out['add_A_halfA']['ms']} ms, lower {out['lower_tri']['ms']} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +243,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_extra:
         extra["convdiff"] = leg_convdiff(fx, P, ctx)
+        extra["setup_algebra"] = leg_setup_algebra(fx, ctx, A, A64)
 
     # ---- 50 weighted-Jacobi sweeps, omega = 2/3 --------------------------
     pj = fx.Precond(ctx, "jacobi", A, 2.0 / 3.0, 1)

@@ -54,6 +54,9 @@ int ensure_dinv32(fx_pc pc) {
 
 Ops::Ops(fx_ctx c, fx_csr a, fx_pc p, int dtype) : ctx(c), A(a), pc(p), dt(dtype), n(a->nrow) {}
 
+Ops::Ops(fx_ctx c, fx_op a, fx_pc p, int dtype)
+    : ctx(c), A(nullptr), pc(p), dt(dtype), n(a->nrow), op(a) {}
+
 Ops::~Ops() {
   for (fx_vec v : owned) fx_vec_destroy(v);
 }
@@ -69,8 +72,24 @@ void Ops::chk(int rc) {
   if (rc != 0) throw OpsError{rc};
 }
 
-void Ops::spmv(fx_vec x, fx_vec y) { chk(fx_spmv(ctx, A, x, y)); }
-void Ops::residual(fx_vec b, fx_vec x, fx_vec r) { chk(fx_residual(ctx, A, b, x, r)); }
+void Ops::spmv(fx_vec x, fx_vec y) {
+  if (!op) {
+    chk(fx_spmv(ctx, A, x, y));
+    return;
+  }
+  const int rc = op->apply(op->user, x, y);
+  if (rc) {
+    if (!fx_last_error()[0]) set_error("operator apply returned %d", rc);
+    throw OpsError{rc < 0 ? rc : FX_ERROR_INPUT_PAR};
+  }
+}
+void Ops::residual(fx_vec b, fx_vec x, fx_vec r) {
+  if (op) {  // LOP::Residual aborts unless a subclass overrides it (src/LOP.hxx:62-65)
+    set_error("the operator has no Residual");
+    throw OpsError{FX_ERROR_INPUT_PAR};
+  }
+  chk(fx_residual(ctx, A, b, x, r));
+}
 void Ops::axpy(double a, fx_vec x, fx_vec y) { chk(fx_axpy(ctx, a, x, y)); }
 void Ops::xpay(double a, fx_vec x, fx_vec y) { chk(fx_xpay(ctx, a, x, y)); }
 void Ops::axpby(double a, double b, fx_vec x, fx_vec y) { chk(fx_axpby(ctx, a, b, x, y)); }
@@ -749,17 +768,53 @@ int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm) {
   // keep the fused-path workspace (and its replay graph) when nothing changed
   const bool same = s->A == A && s->pc == pc && std::memcmp(&np, &s->prm, sizeof(fx_params)) == 0;
   s->A = A;
+  s->op = nullptr;
   s->pc = pc;
   s->prm = np;
   if (!same) s->release_fast();
   return 0;
 }
 
+int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply, void* user,
+                 fx_op* out) {
+  clear_error();
+  FX_CHECK(ctx && apply && out, FX_ERROR_INPUT_PAR, "null argument");
+  fx_op op = new fx_op_s();
+  op->ctx = ctx;
+  op->nrow = nrow;
+  op->mcol = mcol;
+  op->apply = apply;
+  op->user = user;
+  *out = op;
+  return 0;
+}
+
+int fx_op_destroy(fx_op op) {
+  delete op;
+  return 0;
+}
+
+int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm) {
+  clear_error();
+  FX_CHECK(s && A && pc, FX_ERROR_INPUT_PAR, "null argument");
+  FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "solvers need a square operator");
+  FX_CHECK(s->type != FX_SOLVER_JACOBI, FX_ERROR_SOLVER_TYPE,
+           "the Jacobi solver needs a matrix (its diagonal)");
+  s->release_fast();
+  s->A = nullptr;
+  s->op = A;
+  s->pc = pc;
+  s->prm = prm ? *prm : s->prm;
+  return 0;
+}
+
 int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
   clear_error();
-  FX_CHECK(s && s->A && b && x && res, FX_ERROR_INPUT_PAR, "solver not set up / null argument");
-  FX_CHECK(b->n == s->A->nrow && x->n == s->A->nrow, FX_ERROR_NONMATCH_SIZE,
-           "Solve: sizes b %u, x %u, A %u", b->n, x->n, s->A->nrow);
+  FX_CHECK(s && (s->A || s->op) && b && x && res, FX_ERROR_INPUT_PAR,
+           "solver not set up / null argument");
+  const uint32_t n = s->op ? s->op->nrow : s->A->nrow;
+  FX_CHECK(b->n == n && x->n == n, FX_ERROR_NONMATCH_SIZE, "Solve: sizes b %u, x %u, A %u", b->n,
+           x->n, n);
   FX_CHECK(b->dtype == FX_F64 && x->dtype == FX_F64, FX_ERROR_INPUT_PAR,
            "solvers take f64 vectors (use fx_solve_ir for f32 inner solves)");
   res->retcode = 0;
@@ -774,10 +829,10 @@ int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
   FX_HIP(hipEventRecord(e0, s->ctx->stream));
   int rc = 0;
   try {
-    if (fast_eligible(s)) {
+    if (!s->op && fast_eligible(s)) {
       rc = fast_solve(s, b, x, res);
     } else {
-      Ops o(s->ctx, s->A, s->pc, FX_F64);
+      Ops o = s->op ? Ops(s->ctx, s->op, s->pc, FX_F64) : Ops(s->ctx, s->A, s->pc, FX_F64);
       switch (s->type) {
         case FX_SOLVER_CG: rc = ops_cg(s, o, b, x, res); break;
         case FX_SOLVER_BICGSTAB: rc = ops_bicgstab(s, o, b, x, res); break;

@@ -11,10 +11,18 @@ struct FastWs;  // fused-path workspace (fx_fast.hip)
 
 }  // namespace fx
 
+struct fx_op_s {
+  fx_ctx ctx = nullptr;
+  uint32_t nrow = 0, mcol = 0;
+  fx_op_apply_fn apply = nullptr;
+  void* user = nullptr;
+};
+
 struct fx_solver_s {
   fx_ctx ctx = nullptr;
   int type = FX_SOLVER_CG;
   fx_csr A = nullptr;
+  fx_op op = nullptr;  // set instead of A by fx_solver_setup_op
   fx_pc pc = nullptr;
   fx_params prm{};
   std::vector<double> hist_host;
@@ -39,9 +47,13 @@ struct Ops {
   fx_pc pc;
   int dt;
   uint32_t n;
+  fx_op op = nullptr;  // the operator's action, when it is not a matrix
   std::vector<fx_vec> owned;
 
   Ops(fx_ctx c, fx_csr a, fx_pc p, int dtype);
+  Ops(fx_ctx c, fx_op a, fx_pc p, int dtype);
+  Ops(const Ops&) = delete;
+  Ops& operator=(const Ops&) = delete;
   ~Ops();
   fx_vec vec();
   static void chk(int rc);

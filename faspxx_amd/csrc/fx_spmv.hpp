@@ -82,6 +82,16 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
 // clamped to its last entry) and branch-free, and the row-pointer loads come
 // after them, so all of a thread's loads are in flight before the first
 // wait.  CD: 16-bit column deltas to r0; VA: the matrix has values.
+// Matrix-stream loads: non-temporal (FX_NT_MATRIX=1) or ordinary.
+#ifndef FX_NT_MATRIX
+#define FX_NT_MATRIX 1
+#endif
+template <class T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  if constexpr (FX_NT_MATRIX) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 template <bool CD, bool VA, class MT, class VT>
 __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, VT>& lds,
                                             uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
@@ -96,10 +106,10 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
     if constexpr (CD)
-      cix[j] = (uint32_t)(int32_t)__builtin_nontemporal_load(&in.cd[k]);
+      cix[j] = (uint32_t)(int32_t)stream_load(&in.cd[k]);
     else
-      cix[j] = __builtin_nontemporal_load(&in.ci[k]);
-    if constexpr (VA) vv[j] = __builtin_nontemporal_load(&in.va[k]);
+      cix[j] = stream_load(&in.ci[k]);
+    if constexpr (VA) vv[j] = stream_load(&in.va[k]);
   }
   constexpr int kRp = kBlockRows / kThreads + 1;  // row pointers per thread
   uint32_t rpv[kRp];

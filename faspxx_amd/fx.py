@@ -128,6 +128,9 @@ _SIGS = {
     "fx_csr_add": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, C.POINTER(_P)]),
     "fx_csr_mult": (C.c_int, [_P, _P, _P, C.POINTER(_P)]),
     "fx_csr_shift_diag": (C.c_int, [_P, _P, C.c_double]),
+    "fx_op_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(_P)]),
+    "fx_op_destroy": (C.c_int, [_P]),
+    "fx_solver_setup_op": (C.c_int, [_P, _P, _P, _P]),
     "fx_csr_pattern_grad": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_pc_create_identity": (C.c_int, [_P, C.POINTER(_P)]),
     "fx_pc_create_mcgs": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
@@ -550,6 +553,55 @@ class MCGS:
                 pass
 
 
+_APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+class _View(Vec):
+    """A borrowed device vector handed to an Operator's apply (not owned)."""
+
+    def __init__(self, ctx, h, n):  # noqa: D401 -- no allocation
+        self.h, self.ctx, self.n, self.dtype = C.c_void_p(h), ctx, n, F64
+
+    def close(self):
+        self.h = None
+
+    def __del__(self):
+        pass
+
+
+class Operator:
+    """A linear operator given by its action, LOP with a user Apply
+    (src/LOP.hxx:56-59): apply(x: Vec, y: Vec) computes y = A x on the
+    context's stream.  An exception in apply becomes ERROR_INPUT_PAR."""
+
+    def __init__(self, ctx: Context, n: int, apply):
+        self.ctx, self.n, self._apply = ctx, n, apply
+        self.error = None
+
+        def tramp(user, x, y):
+            try:
+                apply(_View(ctx, x, n), _View(ctx, y, n))
+                return 0
+            except Exception as e:  # reported through the return code
+                self.error = e
+                return -12
+
+        self._fn = _APPLY_FN(tramp)
+        h = C.c_void_p()
+        _chk(load().fx_op_create(ctx.h, n, n, C.cast(self._fn, C.c_void_p), None, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def close(self):
+        if self.h:
+            load().fx_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if not _exiting:
+            self.close()
+
+
 class Solver:
     def __init__(self, ctx: Context, kind: int):
         h = C.c_void_p()
@@ -558,9 +610,14 @@ class Solver:
         self._keep = ()
         ctx._adopt(self)
 
-    def Setup(self, A: Csr, pc: Precond, prm: Params):
+    def Setup(self, A, pc: Precond, prm: Params):
+        """A: a Csr, or an Operator (its apply runs where the reference calls
+        A->Apply; the per-operation solver form)."""
         self._keep = (A, pc)
-        _chk(load().fx_solver_setup(self.h, A.h, pc.h, C.byref(prm)))
+        if isinstance(A, Operator):
+            _chk(load().fx_solver_setup_op(self.h, A.h, pc.h, C.byref(prm)))
+        else:
+            _chk(load().fx_solver_setup(self.h, A.h, pc.h, C.byref(prm)))
 
     def Solve(self, b: Vec, x: Vec) -> Result:
         """Returns the Result; retcode carries FaspRetCode (negative codes such

@@ -47,6 +47,10 @@ void VEC::Resize(USI n) {
     size = n;
     return;
   }
+  if (borrowed) {
+    std::fprintf(stderr, "### ABORT: a vector handed to LOP::Apply cannot change size\n");
+    std::abort();
+  }
   if (dev) fx_vec_destroy(dev);
   dev = nullptr;
   check(fx_vec_create(Context(), n, FX_F64, &dev), "fx_vec_create");
@@ -98,8 +102,10 @@ VEC::VEC(const VEC& src) : size(0) {
   check(fx_vec_copy(src.Device(), dev), "fx_vec_copy");
 }
 
+VEC::VEC(fx_vec view, bool b) : size(fx_vec_size(view)), borrowed(b), dev(view) {}
+
 VEC::~VEC() {
-  if (dev) fx_vec_destroy(dev);
+  if (dev && !borrowed) fx_vec_destroy(dev);
 }
 
 VEC& VEC::operator=(const VEC& src) {
@@ -631,7 +637,33 @@ FaspRetCode Jacobi::Solve(const VEC& b, VEC& x) {
 // ---------------------------------------------------------------------------
 // Krylov solvers
 // ---------------------------------------------------------------------------
+namespace {
+
+// y = A x for an operator that is not a MAT: its own Apply on views of the
+// solver's device vectors; values it wrote through GetArray / operator[]
+// go back to the device before the solver continues.
+int lop_apply(void* user, fx_vec x, fx_vec y) {
+  try {
+    const LOP* L = static_cast<const LOP*>(user);
+    const VEC xv(x, true);
+    VEC yv(y, true);
+    L->Apply(xv, yv);
+    (void)yv.Device();
+    return 0;
+  } catch (...) {
+    return ERROR_INPUT_PAR;
+  }
+}
+
+const MAT& no_matrix() {
+  static const MAT m;
+  return m;
+}
+
+}  // namespace
+
 KrylovSOL::~KrylovSOL() {
+  if (op) fx_op_destroy(op);
   if (solver) fx_solver_destroy(solver);
 }
 
@@ -639,18 +671,15 @@ FaspRetCode KrylovSOL::Setup(const LOP& A) {
   SetSolType(kind);
   this->A = &A;
   mat = dynamic_cast<const MAT*>(&A);
-  if (!mat) {
-    std::cerr << "### WARNING: device solvers need a MAT operator\n";
-    return ERROR_INPUT_PAR;
-  }
+  lop = mat ? nullptr : &A;  // any other LOP: its Apply is called back
   return SUCCESS;
 }
 
 FaspRetCode KrylovSOL::Solve(const VEC& b, VEC& x) {
-  if (!mat) return ERROR_INPUT_PAR;
+  if (!mat && !lop) return ERROR_INPUT_PAR;
   fx_pc p = nullptr;
   if (pcd) {
-    p = pcd->DevicePC(*mat);
+    p = pcd->DevicePC(mat ? *mat : no_matrix());
     if (!p) return ERROR_SOLVER_PCD_TYPE;
   } else {
     if (!pc_cache) check(fx_pc_create_identity(Context(), &pc_cache), "fx_pc_create_identity");
@@ -661,7 +690,17 @@ FaspRetCode KrylovSOL::Solve(const VEC& b, VEC& x) {
   prm.maxRestart = maxRestart;
   prm.minRestart = minRestart;
   prm.leftPrecond = rightPrecond ? 0 : 1;
-  int rc = fx_solver_setup(solver, mat->Device(), p, &prm);
+  int rc = 0;
+  if (mat) {
+    rc = fx_solver_setup(solver, mat->Device(), p, &prm);
+  } else {
+    if (op) fx_op_destroy(op);
+    op = nullptr;
+    check(fx_op_create(Context(), lop->GetRowSize(), lop->GetColSize(), lop_apply,
+                       const_cast<LOP*>(lop), &op),
+          "fx_op_create");
+    rc = fx_solver_setup_op(solver, op, p, &prm);
+  }
   fx_result res{};
   if (!rc) rc = fx_solver_solve(solver, b.Device(), x.DeviceOut(), &res);
   if (rc == FX_ERROR_DEVICE) check(rc, "Solve");

@@ -207,9 +207,13 @@ class VEC {
   void DeviceWritten();
   // The device copy, current, about to be overwritten (drops the mirror).
   fx_vec DeviceOut();
+  // A view of a device vector owned elsewhere (no reference counterpart):
+  // the solvers hand these to a LOP's Apply.  Its size cannot change.
+  VEC(fx_vec view, bool borrowed);
 
  private:
   USI size;
+  bool borrowed = false;
   mutable fx_vec dev = nullptr;
   mutable std::vector<DBL> host;
   mutable bool host_valid = false;    // mirror holds the current values
@@ -388,6 +392,8 @@ class KrylovSOL : public SOL {
  protected:
   SOLType kind;
   const MAT* mat = nullptr;
+  const LOP* lop = nullptr;  // an operator that is not a MAT: its Apply is called back
+  fx_op op = nullptr;
   fx_solver solver = nullptr;
   uint32_t maxRestart = 30, minRestart = 10;
   bool rightPrecond = true;

@@ -35,6 +35,13 @@ typedef struct fx_pc_s* fx_pc;
 typedef struct fx_solver_s* fx_solver;
 typedef struct fx_ir_s* fx_ir;
 typedef struct fx_mcgs_s* fx_mcgs;
+typedef struct fx_op_s* fx_op;
+
+/* y = A x for a callback operator: called on the host with the context's
+ * stream current; it enqueues its device work on that stream (or works on
+ * host copies it synchronises itself) and returns 0 or a negative
+ * FaspRetCode. */
+typedef int (*fx_op_apply_fn)(void* user, fx_vec x, fx_vec y);
 
 /* Value types of vectors and matrix values. */
 enum { FX_F64 = 0, FX_F32 = 1 };
@@ -271,6 +278,16 @@ int fx_solver_create(fx_ctx ctx, int type, fx_solver* out);
  * For FX_SOLVER_JACOBI, pc must be a Jacobi preconditioner; its weight is
  * the solver's weight and its iteration count is ignored. */
 int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm);
+/* A linear operator given by its action: a LOP subclass with its own Apply
+ * (src/LOP.hxx:22-66; the matrix-free pattern of example/TestMatFree.cxx).
+ * The Krylov solvers (CG, BiCGStab, GMRES, FGMRES) accept it through
+ * fx_solver_setup_op with any preconditioner and run their per-operation
+ * form, calling apply wherever the reference calls A->Apply.  The Jacobi
+ * solver needs a matrix (its diagonal), as in the reference. */
+int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply, void* user,
+                 fx_op* out);
+int fx_op_destroy(fx_op op);
+int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm);
 /* Solve(b, x): x is the initial guess on entry (CG::Solve, src/CG.cxx:53-249;
  * BiCGStab::Solve, src/BiCGStab.cxx:68-270; GMRES::Solve, src/GMRES.cxx:16-22;
  * FGMRES::Solve, src/FGMRES.cxx:108-271; Jacobi::Solve, src/Iter.cxx:56-112).

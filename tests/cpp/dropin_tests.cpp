@@ -197,6 +197,76 @@ static void test_mat_algebra() {
   std::printf("ok mat algebra\n");
 }
 
+// example/TestMatFree.cxx's pattern: a LOP subclass whose Apply works on
+// host copies (GetArray) -- the 5-point Laplacian on an m x m grid -- solved
+// by the device CG; against the same solve with the assembled MAT.
+class MatFree : public LOP {
+ public:
+  explicit MatFree(USI m) : LOP(m * m), m(m) {}
+  void Apply(const VEC& x, VEC& y) const override {
+    const DBL* xv;
+    x.GetArray(&xv);
+    y.SetValues(x.GetSize(), 0.0);
+    DBL* yv;
+    y.GetArray(&yv);
+    for (USI r = 0; r < m; ++r)
+      for (USI c = 0; c < m; ++c) {
+        const USI k = r * m + c;
+        DBL v = 4.0 * xv[k];
+        if (r > 0) v -= xv[k - m];
+        if (c > 0) v -= xv[k - 1];
+        if (c + 1 < m) v -= xv[k + 1];
+        if (r + 1 < m) v -= xv[k + m];
+        yv[k] = v;
+      }
+  }
+
+ private:
+  USI m;
+};
+
+static void test_matfree() {
+  const USI m = 31, n = m * m;
+  std::vector<DBL> va;
+  std::vector<USI> ci, rp(1, 0);
+  for (USI r = 0; r < m; ++r)
+    for (USI c = 0; c < m; ++c) {
+      const USI k = r * m + c;
+      if (r > 0) ci.push_back(k - m), va.push_back(-1.0);
+      if (c > 0) ci.push_back(k - 1), va.push_back(-1.0);
+      ci.push_back(k), va.push_back(4.0);
+      if (c + 1 < m) ci.push_back(k + 1), va.push_back(-1.0);
+      if (r + 1 < m) ci.push_back(k + m), va.push_back(-1.0);
+      rp.push_back((USI)ci.size());
+    }
+  const MAT A(n, n, (USI)ci.size(), va, ci, rp);
+  std::vector<DBL> bh(n);
+  for (USI k = 0; k < n; ++k) bh[k] = std::sin(0.37 * k);
+  const VEC b(bh);
+  VEC x1(n, 0.25), x2(n, 0.25);  // TestMatFree's initial guess
+  Identity pcd;
+  MatFree op(m);
+  USI it[2];
+  for (int pass = 0; pass < 2; ++pass) {
+    CG cg;
+    cg.SetMaxIter(10000);
+    cg.SetRelTol(1e-8);
+    cg.SetAbsTol(1e-12);
+    cg.SetupPCD(pcd);
+    if (pass == 0)
+      cg.Setup(op);
+    else
+      cg.Setup(A);
+    REQUIRE(cg.Solve(b, pass == 0 ? x1 : x2) == SUCCESS);
+    it[pass] = cg.GetIterations();
+  }
+  REQUIRE(it[0] > 10 && (it[0] > it[1] ? it[0] - it[1] : it[1] - it[0]) <= 2);
+  VEC d(x1);
+  d.AXPY(-1.0, x2);
+  REQUIRE(d.Norm2() <= 1e-8 * x2.Norm2());
+  std::printf("ok matfree (%u / %u iterations)\n", it[0], it[1]);
+}
+
 static void test_jacobi() {
   // UnitTestsJacobi.cxx:85-157 (3x3, Octave solution, tol 1e-5)
   std::vector<DBL> value = {4, 2, 1, 3, 8, 2, 2, 1, 9};
@@ -337,6 +407,7 @@ int main(int argc, char** argv) {
   test_vec();
   test_mat();
   test_mat_algebra();
+  test_matfree();
   test_jacobi();
   test_solvers();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);

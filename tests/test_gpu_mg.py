@@ -71,13 +71,14 @@ def test_testmg_sample(oracle):
 @pytest.mark.parametrize("levels,ncycles", [(3, 1), (3, 2), (4, 1)])
 def test_one_sweep_solver(oracle, tmp_path, levels, ncycles):
     """One-sweep smoothers and coarse solvers, a random right-hand side, MG
-    iterated to relTol 1e-10."""
-    A = csrx(GOLD / "fem_small.csr", oracle)
+    iterated to relTol 1e-10 (fdm_10X10: a cycle is 2L + 1 Jacobi sweeps
+    here, too few for fem_small's 3969 unknowns within 400 cycles)."""
+    A = csrx(GOLD / "fdm_10X10.csr", oracle)
     rng = np.random.default_rng(levels * 10 + ncycles)
     b = rng.standard_normal(A.nrow)
     write_vec(tmp_path / "b.vec", b)
     write_vec(tmp_path / "x0.vec", np.zeros(A.nrow))
-    rc, it, n2, x = run_mg(GOLD / "fem_small.csr", levels, ncycles, "one", 400, 0, 1e-10, 0.0,
+    rc, it, n2, x = run_mg(GOLD / "fdm_10X10.csr", levels, ncycles, "one", 400, 0, 1e-10, 0.0,
                            ["-", tmp_path / "b.vec", tmp_path / "x0.vec"])
     o = oracle.mg(A, b, np.zeros(A.nrow), [(None, ONE, ONE)] * levels, ncycles=ncycles,
                   maxIter=400, relTol=1e-10, absTol=0.0)
@@ -95,7 +96,7 @@ def test_square_transfer_operator(oracle, tmp_path):
     rng = np.random.default_rng(11)
     Td = np.eye(n) * 0.7 + (rng.random((n, n)) < 0.03) * rng.uniform(-0.2, 0.2, (n, n))
     r, c = np.nonzero(Td)
-    body = "\n".join(f"{i + 1} {j + 1} {Td[i, j]!r}" for i, j in zip(r, c))
+    body = "\n".join(f"{i + 1} {j + 1} {float(Td[i, j])!r}" for i, j in zip(r, c))
     (tmp_path / "T.mtx").write_text(
         f"%%MatrixMarket matrix coordinate real general\n{n} {n} {len(r)}\n{body}\n")
     T = csrx(tmp_path / "T.mtx", oracle)
