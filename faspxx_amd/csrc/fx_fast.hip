@@ -923,8 +923,10 @@ bool fast_eligible(fx_solver S) {
   const bool pc_ok =
       S->pc->kind == 0 || (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A == S->A);
   if (S->type == FX_SOLVER_CG || S->type == FX_SOLVER_BICGSTAB) return pc_ok;
-  // GMRES: the safe-guard copy (savIter <= maxIter) stays on the per-operation path
-  if (S->type == FX_SOLVER_GMRES) return pc_ok && S->prm.savIter > S->prm.maxIter;
+  // GMRES / FGMRES: the safe-guard copy (savIter <= maxIter) stays on the
+  // per-operation path
+  if (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
+    return pc_ok && S->prm.savIter > S->prm.maxIter;
   if (S->type == FX_SOLVER_JACOBI) return S->pc->A == S->A;
   return false;
 }
@@ -934,7 +936,8 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
     S->fast = new FastWs();
     const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacState)
                       : S->type == FX_SOLVER_BICGSTAB ? bicgstab_state_bytes()
-                      : S->type == FX_SOLVER_GMRES    ? gmres_state_bytes()
+                      : (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
+                          ? gmres_state_bytes()
                                                       : sizeof(CgState);
     int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
                       spmv_grid(S->A->nblk, S->ctx->sms));
@@ -948,7 +951,8 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
   if (S->type == FX_SOLVER_JACOBI) return jacobi_run(S, W, (const double*)b->d, (double*)x->d, res);
   if (S->type == FX_SOLVER_BICGSTAB)
     return bicgstab_fast(S, W, (const double*)b->d, (double*)x->d, res);
-  if (S->type == FX_SOLVER_GMRES) return gmres_fast(S, W, (const double*)b->d, (double*)x->d, res);
+  if (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
+    return gmres_fast(S, W, (const double*)b->d, (double*)x->d, res);
   CgRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
                     S->ctx->stream};
   return run.run((const double*)b->d, (double*)x->d, res, S->prm, FX_F64);

@@ -20,7 +20,12 @@
 //             their norms; the state goes to a pinned host slot   (k_spmv + k_finish)
 // The host reads the state once per cycle and applies the variable-restart
 // rule of :256-265 to choose the next cycle's graph (one per length, made on
-// first use).  Modified Gram-Schmidt keeps its sequential order: one kernel
+// first use).  FGMRES (FGMRES::Solve, src/FGMRES.cxx:108-271) runs the same
+// cycle with the preconditioned vectors kept: Z[k] = P(V[k]) from the stale
+// Z[k] (one Jacobi sweep, z + D^-1 (v - A z), with the reference's shortcut
+// v when ||z||_inf < 1e-20; identity: Z[k] = V[k]), w = A Z[k], and
+// x += sum_j y_j Z[j]; ||Z[k]||_inf rides along as a second reduction slot of
+// the SpMV that writes Z[k].  Modified Gram-Schmidt keeps its sequential order: one kernel
 // per step, as the method needs one reduction per step.  The Hessenberg
 // matrix, the rotations and the right-hand side g stay in f64 on the device;
 // the arithmetic is the reference's (src/GMRES.cxx:182-219).
@@ -109,7 +114,7 @@ template <class VT>
 __global__ __launch_bounds__(kWideThreads) void k_gm_mgs(
     VT* __restrict__ w, const VT* __restrict__ vj, const VT* __restrict__ next, uint32_t n,
     const GmState* st, const double* __restrict__ pin, unsigned gin, double* __restrict__ pout,
-    GmLsq L, uint32_t j, uint32_t k) {
+    GmLsq L, uint32_t j, uint32_t k, double* zinf_out) {
   if (st->status != GM_RUN) return;
   using PK = Pack16<VT>;
   constexpr int W = PK::W;
@@ -121,8 +126,16 @@ __global__ __launch_bounds__(kWideThreads) void k_gm_mgs(
   PK wa = ld16(w, c0), va = ld16(vj, c0), wb = ld16(w, c1), vb = ld16(vj, c1);
   PK na = self ? PK{} : ld16(next, c0), nb = self ? PK{} : ld16(next, c1);
   double h[1];
-  partials_thread<1, 0u, kWideThreads>(pin, gin, threadIdx.x, h);
-  partials_combine<1, 0u, kWideThreads>(h);
+  if (zinf_out) {  // FGMRES, j = 0: the partials also hold max |Z[k]| (slot 1)
+    double hz[2];
+    partials_thread<2, 2u, kWideThreads>(pin, gin, threadIdx.x, hz);
+    partials_combine<2, 2u, kWideThreads>(hz);
+    h[0] = hz[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zinf_out = hz[1];
+  } else {
+    partials_thread<1, 0u, kWideThreads>(pin, gin, threadIdx.x, h);
+    partials_combine<1, 0u, kWideThreads>(h);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) L.H[(size_t)j * L.ld + k] = h[0];
   // w = AXPBY(1, -h, V[j], w)   (src/GMRES.cxx:170-171; VEC::AXPBY forms)
   const VT b = (VT)(-h[0]);
@@ -293,11 +306,100 @@ struct FinGmRes {  // rj = ||b - A x||, resAbs = ||V0||, state -> host slot
   }
 };
 
+// ---- FGMRES ----------------------------------------------------------------
+// P_k (Jacobi): tmp = z + (work .* dinv), work = v - A z, or v when the stale
+// z has ||z||_inf < 1e-20 (Jacobi::Solve as the preconditioner,
+// src/Iter.cxx:69-99; MAT::Residual's shortcut, src/MAT.cxx:459-462)
+template <class VT>
+struct EpiFgmPc {
+  VT* __restrict__ tmp;
+  const VT* __restrict__ z;
+  const VT* __restrict__ v;
+  const VT* __restrict__ dinv;
+  const double* zinf;
+  const GmState* st;
+  __device__ bool skip() const { return st->status != GM_RUN; }
+  __device__ __forceinline__ void row(uint32_t i, double r, double (&)[1]) const {
+    const VT work = (*zinf < kCloseZero) ? v[i] : (VT)r;
+    const VT wd = work * dinv[i];  // work.PointwiseMult(diagInv)
+    tmp[i] = z[i] + wd;            // x += work
+  }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+// S_k: w = A z with z = tmp (Jacobi) or V[k] (identity); Z[k] = z; partials
+// of (w, V0) and max |z|
+template <class VT>
+struct EpiFgmW {
+  VT* __restrict__ w;
+  VT* __restrict__ zk;
+  const VT* __restrict__ z;
+  const VT* __restrict__ v0;
+  const GmState* st;
+  __device__ bool skip() const { return st->status != GM_RUN; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
+    const VT a = (VT)v;
+    w[i] = a;
+    const VT zi = z[i];
+    zk[i] = zi;
+    acc[0] = fma((double)a, (double)v0[i], acc[0]);
+    acc[1] = fmax(acc[1], fabs((double)zi));
+  }
+  __device__ void fin(double (&)[2]) const {}
+};
+
+// X: wk = sum_j y_j Z[j] (src/FGMRES.cxx:216-221), x += wk
+template <class VT>
+__global__ __launch_bounds__(kWideThreads) void k_fgm_update(VT* __restrict__ x,
+                                                             const VT* const* __restrict__ Z,
+                                                             uint32_t n, const GmState* st,
+                                                             GmLsq L) {
+  const uint32_t cnt = st->count;
+  if (cnt == 0) return;
+  const uint32_t stride = gridDim.x * kWideThreads;
+  for (uint32_t i = blockIdx.x * kWideThreads + threadIdx.x; i < n; i += stride) {
+    VT wk = Z[cnt - 1][i] * (VT)L.g[cnt - 1];
+    for (int j = (int)cnt - 2; j >= 0; --j) {
+      const double gj = L.g[j];
+      wk = gj == 1.0 ? wk + Z[j][i] : fma((VT)gj, Z[j][i], wk);
+    }
+    x[i] = fma((VT)1, wk, x[i]);
+  }
+}
+
+// ||Z[j]||_inf of every stored Z[j] (blockIdx.y = j): per-workgroup maxima,
+// then one workgroup per j.
+template <class VT>
+__global__ __launch_bounds__(kThreads) void k_zinf_part(const VT* const* __restrict__ Z,
+                                                        uint32_t n, double* __restrict__ part) {
+  const VT* z = Z[blockIdx.y];
+  double m = 0.0;
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    m = fmax(m, fabs((double)z[i]));
+  double a[1] = {m};
+  block_reduce<1, 1u>(a);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = a[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_zinf_fin(const double* __restrict__ part,
+                                                       uint32_t g, double* __restrict__ zinf) {
+  double m = 0.0;
+  for (uint32_t b = threadIdx.x; b < g; b += kThreads)
+    m = fmax(m, part[(size_t)blockIdx.x * g + b]);
+  double a[1] = {m};
+  block_reduce<1, 1u>(a);
+  if (threadIdx.x == 0) zinf[blockIdx.x] = a[0];
+}
+
 // ---------------------------------------------------------------------------
 struct GmWs {
   uint32_t maxR = 0;
   std::vector<void*> V;      // maxR + 1 device vectors
   void** Vdev = nullptr;     // device copy of the pointer table
+  void** Zdev = nullptr;     // FGMRES: the solver's Z vectors' pointer table
+  double* zinf = nullptr;    // FGMRES: ||Z[j]||_inf, maxR + 1 entries
+  double* zpart = nullptr;   // FGMRES: (maxR + 1) x kMaxRedBlocks partial maxima
   void* tmp = nullptr;
   double* lsq = nullptr;     // H, hc, hs, g
   std::map<uint32_t, hipGraphExec_t> cycles;
@@ -305,6 +407,9 @@ struct GmWs {
     for (auto& c : cycles) (void)hipGraphExecDestroy(c.second);
     for (void* v : V) (void)hipFree(v);
     (void)hipFree(Vdev);
+    (void)hipFree(Zdev);
+    (void)hipFree(zinf);
+    (void)hipFree(zpart);
     (void)hipFree(tmp);
     (void)hipFree(lsq);
   }
@@ -319,8 +424,10 @@ struct GmRun {
   const VT* dinv;  // null: identity
   bool left;
   hipStream_t st;
+  bool flex = false;  // FGMRES
 
   GmState* dstate() const { return (GmState*)W.state; }
+  VT* zvec(uint32_t j) const { return (VT*)fx_vec_data(S->Z[j]); }
   GmState* hslot() const { return (GmState*)W.hstate; }
   GmLsq lsq() const {
     const uint32_t m = G.maxR;
@@ -380,7 +487,19 @@ struct GmRun {
     for (uint32_t k = 0; k < m; ++k) {
       VT* w = vec(k + 1);
       const GmState* cst = dstate();
-      if (!left && dinv) {
+      if (flex) {
+        const VT* z = vec(k);  // identity: Z[k] = V[k]
+        if (dinv) {
+          FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 0, 0u>(
+              A, st, XLoad<VT>{zvec(k)}, BLoad<VT>{vec(k)},
+              EpiFgmPc<VT>{(VT*)G.tmp, zvec(k), vec(k), dinv, G.zinf + k, cst}, nullptr, nullptr,
+              nullptr, false)));
+          z = (const VT*)G.tmp;
+        }
+        FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 2u>(A, st, XLoad<VT>{z}, BNone{},
+                                                   EpiFgmW<VT>{w, zvec(k), z, vec(0), cst}, pa,
+                                                   nullptr, nullptr, false)));
+      } else if (!left && dinv) {
         FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XScaled<VT>{vec(k), dinv}, BNone{},
                                                    EpiGmW<VT, false>{w, vec(0), dinv, cst}, pa,
                                                    nullptr, nullptr, false)));
@@ -398,8 +517,10 @@ struct GmRun {
       for (uint32_t j = 0; j <= k; ++j) {
         double* pout = (pin == pa) ? pb : pa;
         const VT* next = (j < k) ? vec(j + 1) : nullptr;
+        const bool zslot = flex && j == 0;  // S_k's partials carry max |Z[k]|
         hipLaunchKernelGGL((k_gm_mgs<VT>), dim3(g), dim3(kWideThreads), 0, st, w,
-                           (const VT*)vec(j), next, n, cst, pin, gin, pout, L, j, k);
+                           (const VT*)vec(j), next, n, cst, pin, gin, pout, L, j, k,
+                           zslot ? G.zinf + k : nullptr);
         FX_HIP(hipGetLastError());
         pin = pout;
         gin = g;
@@ -408,7 +529,10 @@ struct GmRun {
                          pin, gin, L, P, k, W.hist);
       FX_HIP(hipGetLastError());
     }
-    if (!left && dinv)
+    if (flex)
+      hipLaunchKernelGGL((k_fgm_update<VT>), dim3(g), dim3(kWideThreads), 0, st, x,
+                         (const VT* const*)G.Zdev, n, (const GmState*)dstate(), L);
+    else if (!left && dinv)
       hipLaunchKernelGGL((k_gm_update<VT, true>), dim3(g), dim3(kWideThreads), 0, st, x,
                          (const VT* const*)G.Vdev, dinv, n, (const GmState*)dstate(), L);
     else
@@ -445,6 +569,14 @@ struct GmRun {
     clamp_restart(PP, maxR, minR, restart);
     VT* x = (VT*)W.x;
     FX_HIP(hipMemcpyAsync(x, x_user, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
+    if (flex) {  // the stale Z of earlier solves: their max-norms
+      const unsigned gz = stream_blocks(n);
+      hipLaunchKernelGGL((k_zinf_part<VT>), dim3(gz, maxR + 1), dim3(kThreads), 0, st,
+                         (const VT* const*)G.Zdev, n, G.zpart);
+      hipLaunchKernelGGL(k_zinf_fin, dim3(maxR + 1), dim3(kThreads), 0, st,
+                         (const double*)G.zpart, gz, G.zinf);
+      FX_HIP(hipGetLastError());
+    }
     // graphs capture b and x: the solve works on the workspace copies
     VT* bw = (VT*)W.r;
     FX_HIP(hipMemcpyAsync(bw, b, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
@@ -477,7 +609,7 @@ struct GmRun {
       if ((resRel < PP.relTol || resAbs < PP.absTol) && s.numIter >= PP.minIter) break;  // :255
       const double cr = rj / ri;
       ri = rj;
-      adapt_restart(cr, false, maxR, minR, restart);
+      adapt_restart(cr, flex, maxR, minR, restart);  // FGMRES: <= / >= (src/FGMRES.cxx:251)
     }
     res->numIter = s.numIter;
     if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
@@ -525,8 +657,24 @@ int gmres_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result* re
     FX_HIP(hipMemset(N.lsq, 0, sizeof(double) * nl));
   }
   GmWs& GG = *(GmWs*)W.gm;
+  const bool flex = S->type == FX_SOLVER_FGMRES;
+  if (flex) {
+    // FGMRES keeps Z in the solver (shared with the per-operation path, so
+    // the stale Z carries over between solves as in the reference)
+    if (S->Z.size() != maxR + 1) {
+      for (fx_vec z : S->Z) fx_vec_destroy(z);
+      S->Z.assign(maxR + 1, nullptr);
+      for (auto& z : S->Z) FX_TRY(fx_vec_create(S->ctx, W.n, FX_F64, &z));
+    }
+    if (!GG.Zdev) FX_HIP(hipMalloc(&GG.Zdev, sizeof(void*) * (maxR + 1)));
+    if (!GG.zinf) FX_HIP(hipMalloc(&GG.zinf, sizeof(double) * (maxR + 1)));
+    if (!GG.zpart) FX_HIP(hipMalloc(&GG.zpart, sizeof(double) * (maxR + 1) * kMaxRedBlocks));
+    std::vector<void*> zp(maxR + 1);
+    for (uint32_t j = 0; j <= maxR; ++j) zp[j] = fx_vec_data(S->Z[j]);
+    FX_HIP(hipMemcpy(GG.Zdev, zp.data(), sizeof(void*) * (maxR + 1), hipMemcpyHostToDevice));
+  }
   GmRun<double> run{S, W, GG, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr,
-                    S->prm.leftPrecond != 0, S->ctx->stream};
+                    !flex && S->prm.leftPrecond != 0, S->ctx->stream, flex};
   return run.run(b, x, res);
 }
 

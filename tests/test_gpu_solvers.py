@@ -261,20 +261,58 @@ def test_gmres_convdiff_full_size(ctx, problems):
     assert abs(out[0] - out[1]) <= 2, out
 
 
-def test_fgmres_parity(ctx, oracle, problems):
+@pytest.mark.parametrize("force_ops", [False, True])
+def test_fgmres_parity(ctx, oracle, problems, force_ops):
+    """Fused cycle graphs (default) and per-operation path against the oracle."""
     pr = problems(3, 48)
     M = Matrix.of(pr.A)
-    o = oracle.solve(FGMRES, M, pr.b, np.zeros(M.nrow), pc="identity", **PRM)
-    res, x, _ = gpu_solve(ctx, pr, fx.FGMRES, "identity", **PRM)
+    o = oracle.solve(FGMRES, M, pr.b, np.zeros(M.nrow), pc="identity", hist_cap=4000, **PRM)
+    res, x, hist = gpu_solve(ctx, pr, fx.FGMRES, "identity", force_ops=force_ops, **PRM)
     assert abs(res.numIter - o["numIter"]) <= 2
     assert rel_err(x, o["x"]) < 1e-6
+    k = min(len(hist), 20)
+    assert k > 5 and rel_err(hist[:k], o["hist"][:k]) < 1e-8
     # Jacobi with the reference's stale initial guess (src/FGMRES.cxx:158-159):
     # same trajectory for a fixed iteration budget
     prm = dict(maxIter=200, relTol=1e-12, absTol=0.0)
     o = oracle.solve(FGMRES, M, pr.b, np.zeros(M.nrow), pc="jacobi", **prm)
-    res, x, _ = gpu_solve(ctx, pr, fx.FGMRES, "jacobi", **prm)
+    res, x, _ = gpu_solve(ctx, pr, fx.FGMRES, "jacobi", force_ops=force_ops, **prm)
     assert res.numIter == o["numIter"] == 200
     assert abs(res.norm2 - o["norm2"]) <= 1e-6 * o["norm2"]
+
+
+def test_fgmres_stale_z_across_solves(ctx, problems):
+    """The reference keeps Z in the solver, so a second Solve starts its
+    Jacobi applications from the first solve's Z: the fused path and the
+    per-operation path carry it over alike."""
+    pr = problems(3, 48)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "jacobi", A, 1.0, 1)
+    prm = fx.params(maxIter=150, relTol=1e-12, absTol=0.0)
+    out = []
+    for force_ops in (False, True):
+        s = fx.Solver(ctx, fx.FGMRES)
+        s.Setup(A, P, prm)
+        norms = []
+        for _ in range(2):
+            x = dev(ctx, np.zeros(pr.A.nrow))
+            old = os.environ.get("FASPXX_FX_NO_FAST")
+            if force_ops:
+                os.environ["FASPXX_FX_NO_FAST"] = "1"
+            try:
+                r = s.Solve(dev(ctx, pr.b), x)
+            finally:
+                if force_ops:
+                    if old is None:
+                        del os.environ["FASPXX_FX_NO_FAST"]
+                    else:
+                        os.environ["FASPXX_FX_NO_FAST"] = old
+            norms.append(r.norm2)
+        out.append(norms)
+        s.close()
+    assert out[0][0] != out[0][1]  # the second solve starts from a different Z
+    for a, b in zip(out[0], out[1]):
+        assert abs(a - b) <= 1e-6 * b
 
 
 def test_krylov_front_end(ctx, problems):
