@@ -95,7 +95,8 @@ __global__ __launch_bounds__(kThreads) void k_convert(const S* __restrict__ s, D
 // Reductions: kind 0 dot(x,y), 1 sum x^2 (then sqrt), 2 max |x|, 3 max x
 // (from SMALL_DBL = -1e60 as VEC::Max does, src/VEC.cxx:382-400), 4 min x
 // (from LARGE_DBL = 1e60, VEC::Min, :403-421; computed as -max(-x)).  Optional
-// fused update y += a x before the reduction (kinds 0/1 with `upd`).
+// fused update before the reduction (kinds 0/1): upd 1 y += a x (AXPY),
+// upd 2 y = a y + x (XPAY).
 // Products are formed in double; sums accumulate in double with fma.  Each
 // workgroup writes one partial; k_finish combines them.
 template <class T>
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int kind, int upd, double a
     } else if (kind == 4) {
       acc[0] = fmax(acc[0], -(double)x[i]);
     } else if (upd) {
-      const T yn = fma((T)a, x[i], y[i]);
+      const T yn = upd == 1 ? fma((T)a, x[i], y[i]) : fma((T)a, y[i], x[i]);
       y[i] = yn;
       const double u = (double)yn;
       acc[0] = fma(u, kind == 0 ? (double)z[i] : u, acc[0]);
@@ -473,6 +474,12 @@ int fx_axpy_norm2(fx_ctx ctx, double a, fx_vec x, fx_vec y, double* out) {
   clear_error();
   FX_CHECK(y, FX_ERROR_INPUT_PAR, "null vector");
   FX_TRY(reduce_dispatch(ctx, 1, 1, a, x, y, nullptr, ctx->dscal));
+  return fetch_scalar(ctx, 0, out);
+}
+int fx_xpay_norm2(fx_ctx ctx, double a, fx_vec x, fx_vec y, double* out) {
+  clear_error();
+  FX_CHECK(y, FX_ERROR_INPUT_PAR, "null vector");
+  FX_TRY(reduce_dispatch(ctx, 1, 2, a, x, y, nullptr, ctx->dscal));
   return fetch_scalar(ctx, 0, out);
 }
 int fx_axpy_dot(fx_ctx ctx, double a, fx_vec x, fx_vec y, fx_vec z, double* out) {

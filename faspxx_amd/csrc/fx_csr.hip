@@ -232,6 +232,13 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp) {
   hipStream_t st = A->ctx->stream;
   FX_HIP(hipMemcpyAsync(A->blk, blk.data(), sizeof(uint32_t) * blk.size(), hipMemcpyHostToDevice,
                         st));
+  std::vector<uint16_t> rpb(A->nrow ? A->nrow : 1, 0);
+  for (size_t b = 0; b + 1 < rows.size(); ++b)
+    for (uint32_t r = rows[b]; r < rows[b + 1]; ++r)
+      rpb[r] = (uint16_t)(host_rp[r] - host_rp[rows[b]]);
+  FX_HIP(hipMalloc(&A->rpb, sizeof(uint16_t) * rpb.size()));
+  FX_HIP(hipMemcpyAsync(A->rpb, rpb.data(), sizeof(uint16_t) * rpb.size(), hipMemcpyHostToDevice,
+                        st));
   // 2-byte column deltas when every block admits them (FASPXX_FX_COL32=1 keeps u32 columns)
   const char* e = std::getenv("FASPXX_FX_COL32");
   if (A->nnz && A->nblk && !(e && e[0] == '1')) {
@@ -268,6 +275,7 @@ void csr_free(fx_csr A) {
   (void)hipFree(A->va32);
   (void)hipFree(A->blk);
   (void)hipFree(A->cd);
+  (void)hipFree(A->rpb);
   (void)hipFree(A->src);
   delete A;
 }

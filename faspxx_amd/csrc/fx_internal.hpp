@@ -108,6 +108,10 @@ struct fx_csr_s {
   // when every block's columns lie within [r0 - 32768, r0 + 32767], as for
   // banded and stencil matrices; 2 bytes per entry instead of 4.
   int16_t* cd = nullptr;
+  // Row pointers relative to the row's block's first entry, 2 bytes each
+  // (a block holds at most kBlockNnz entries or one row): what the SpMV
+  // kernels read in place of rp.
+  uint16_t* rpb = nullptr;
   // set on a transpose: entry p of this matrix is entry src[p] of its source
   uint32_t* src = nullptr;
 };

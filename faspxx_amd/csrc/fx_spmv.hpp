@@ -13,7 +13,8 @@
 //     w = fma(vk, xk, w)), so the result equals the reference's FMA-contracted
 //     build bit for bit; for banded / stencil matrices a wave's k-th gathers
 //     then hit consecutive x entries.  Blocks of longer rows give each row a
-//     lane group of 2..64 lanes (butterfly sum);
+//     lane group of 2..64 lanes (butterfly sum), and a row of more than
+//     kLongRow entries among short ones is summed by the whole workgroup;
 //   * a row longer than the LDS stage gets a whole workgroup.
 // The epilogue receives (row, value) and may store, update other vectors and
 // add to per-thread reduction slots; the slots are then reduced over the grid
@@ -31,6 +32,7 @@ enum { ROW_AX = 0, ROW_B_MINUS_AX = 1 };
 template <class MT, class VT>
 struct SpmvIn {
   const uint32_t* __restrict__ rp;
+  const uint16_t* __restrict__ rpb;  // row pointers relative to the block's first entry
   const uint32_t* __restrict__ ci;
   const int16_t* __restrict__ cd;  // column deltas to the block's first row, or null
   const MT* __restrict__ va;  // null: structure-only matrix (values are 1)
@@ -44,11 +46,12 @@ inline SpmvIn<MT, VT> spmv_in(fx_csr A);
 
 template <>
 inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
-  return {A->rp, A->ci, A->cd, A->has_values ? A->va : nullptr, A->blk, A->nblk, A->nnz};
+  return {A->rp, A->rpb, A->ci, A->cd, A->has_values ? A->va : nullptr, A->blk, A->nblk, A->nnz};
 }
 template <>
 inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
-  return {A->rp, A->ci, A->cd, A->has_values ? A->va32 : nullptr, A->blk, A->nblk, A->nnz};
+  return {A->rp, A->rpb, A->ci, A->cd, A->has_values ? A->va32 : nullptr, A->blk, A->nblk,
+          A->nnz};
 }
 
 // LDS footprint: row pointers + staged (column, value) pairs.
@@ -65,6 +68,12 @@ struct SpmvLds {
 // that both fits 256/nr and leaves each lane at least kSeqRowLen/2 entries,
 // capped at one wave.
 constexpr uint32_t kSeqRowLen = 32;
+// A row longer than this in a block whose rows are summed one per thread is
+// summed by the whole workgroup instead.
+#ifndef FX_LONG_ROW
+#define FX_LONG_ROW 128
+#endif
+constexpr uint32_t kLongRow = FX_LONG_ROW;
 __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
   if (nr == 0 || nz <= kSeqRowLen * nr) return 1;
   const uint32_t by_rows = kThreads / nr;
@@ -82,6 +91,15 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
 // warm SpMV, and the same time for an SpMV after a cache flush.
 #ifndef FX_NT_MATRIX
 #define FX_NT_MATRIX 0
+#endif
+// The plain SpMV's result: non-temporal stores (FX_NT_Y=1, default) or
+// ordinary ones.
+#ifndef FX_NT_Y
+#define FX_NT_Y 1
+#endif
+// Row pointers: 2-byte block-relative (FX_RP16=1, default) or u32.
+#ifndef FX_RP16
+#define FX_RP16 1
 #endif
 template <class T>
 __device__ __forceinline__ T stream_load(const T* p) {
@@ -114,13 +132,24 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
       cix[j] = stream_load(&in.ci[k]);
     if constexpr (VA) vv[j] = stream_load(&in.va[k]);
   }
+#if FX_RP16
+  // 2-byte block-relative row pointers; the block's end is nz
+  constexpr int kRp = kBlockRows / kThreads;  // row pointers per thread
+  uint32_t rpv[kRp];
+#pragma unroll
+  for (int j = 0; j < kRp; ++j) {
+    const uint32_t i = tid + j * kThreads;
+    rpv[j] = in.rpb[r0 + (i < nr ? i : nr - 1)];
+  }
+#else
   constexpr int kRp = kBlockRows / kThreads + 1;  // row pointers per thread
   uint32_t rpv[kRp];
 #pragma unroll
   for (int j = 0; j < kRp; ++j) {
     const uint32_t i = tid + j * kThreads;
-    rpv[j] = in.rp[r0 + (i <= nr ? i : nr)];
+    rpv[j] = in.rp[r0 + (i <= nr ? i : nr)] - k0;
   }
+#endif
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = tid + j * kThreads;
@@ -132,8 +161,15 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
 #pragma unroll
   for (int j = 0; j < kRp; ++j) {
     const uint32_t i = tid + j * kThreads;
-    if (i <= nr) lds.rp[i] = rpv[j] - k0;
+#if FX_RP16
+    if (i < nr) lds.rp[i] = rpv[j];
+#else
+    if (i <= nr) lds.rp[i] = rpv[j];
+#endif
   }
+#if FX_RP16
+  if (tid == 0) lds.rp[nr] = nz;
+#endif
 }
 
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
@@ -195,8 +231,58 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
       __syncthreads();
       const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {
+        // A row longer than kLongRow in a block of short rows is left to the
+        // whole workgroup afterwards (in row order: a ballot prefix gives
+        // each its slot), so that it does not hold the block's other
+        // threads back; it is summed in strides like a lane-group row.
+        uint32_t nlong = 0;
+        if constexpr (kBlockRows == kThreads) {
+          __shared__ uint16_t long_row[kBlockRows];
+          __shared__ uint32_t wave_long[kWaves];
+          const bool is_long = (uint32_t)tid < nr && lds.rp[tid + 1] - lds.rp[tid] > kLongRow;
+          const uint64_t m = __ballot(is_long);
+          const int lane = tid & 63, wid = tid >> 6;
+          if (lane == 0) wave_long[wid] = (uint32_t)__popcll(m);
+          __syncthreads();
+          uint32_t before = 0;
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w) {
+            before += w < wid ? wave_long[w] : 0u;
+            nlong += wave_long[w];
+          }
+          if (is_long) long_row[before + __popcll(m & ((1ull << lane) - 1ull))] = tid;
+          if (nlong) {
+            __syncthreads();
+            for (uint32_t q = 0; q < nlong; ++q) {
+              const uint32_t rr = long_row[q], s = lds.rp[rr], e = lds.rp[rr + 1];
+              double t = 0.0;
+              for (uint32_t kb = s + tid; kb < e; kb += 4 * kThreads) {
+                double a[4], xv[4];
+                uint32_t c[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  const uint32_t k = min(kb + j * kThreads, e - 1);
+                  a[j] = (double)lds.va[k];
+                  c[j] = lds.ci[k];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                  if (kb + j * kThreads < e) t = fma(a[j], xv[j], t);
+              }
+              double tt[1] = {t};
+              block_reduce<1>(tt);
+              if (tid == 0) {
+                const double v = (MODE == ROW_AX) ? tt[0] : (double)bf(r0 + rr) - tt[0];
+                epi.row(r0 + rr, v, acc);
+              }
+            }
+          }
+        }
         for (uint32_t row = tid; row < nr; row += kThreads) {
           const uint32_t s = lds.rp[row], e = lds.rp[row + 1];
+          if (nlong && e - s > kLongRow) continue;  // summed above
           double t = (MODE == ROW_AX) ? 0.0 : (double)bf(r0 + row);
           // chunks of 8: the chunk's LDS reads, then its gathers, are all
           // issued before its fma chain (indices past the row's end are
@@ -279,7 +365,11 @@ struct EpiStore {
   VT* __restrict__ y;
   __device__ bool skip() const { return false; }
   __device__ __forceinline__ void row(uint32_t r, double v, double (&)[1]) const {
+#if FX_NT_Y
     __builtin_nontemporal_store((VT)v, &y[r]);
+#else
+    y[r] = (VT)v;
+#endif
   }
   __device__ void fin(double (&)[1]) const {}
 };

@@ -106,6 +106,7 @@ _SIGS = {
     "fx_dot_dev": (C.c_int, [_P, _P, _P, _P]),
     "fx_norm2_dev": (C.c_int, [_P, _P, _P]),
     "fx_axpy_norm2": (C.c_int, [_P, C.c_double, _P, _P, C.POINTER(C.c_double)]),
+    "fx_xpay_norm2": (C.c_int, [_P, C.c_double, _P, _P, C.POINTER(C.c_double)]),
     "fx_axpy_dot": (C.c_int, [_P, C.c_double, _P, _P, _P, C.POINTER(C.c_double)]),
     "fx_csr_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P,
                                 C.POINTER(_P)]),
@@ -344,6 +345,25 @@ class Vec:
     def NormInf(self) -> float:
         out = C.c_double()
         _chk(load().fx_norminf(self.ctx.h, self.h, C.byref(out)))
+        return out.value
+
+    # fused forms (no reference counterpart: one pass instead of two)
+    def AXPY_Norm2(self, a: float, x: "Vec") -> float:
+        """this += a x, then ||this||."""
+        out = C.c_double()
+        _chk(load().fx_axpy_norm2(self.ctx.h, a, x.h, self.h, C.byref(out)))
+        return out.value
+
+    def XPAY_Norm2(self, a: float, x: "Vec") -> float:
+        """this = a this + x, then ||this||."""
+        out = C.c_double()
+        _chk(load().fx_xpay_norm2(self.ctx.h, a, x.h, self.h, C.byref(out)))
+        return out.value
+
+    def AXPY_Dot(self, a: float, x: "Vec", z: "Vec") -> float:
+        """this += a x, then (this, z)."""
+        out = C.c_double()
+        _chk(load().fx_axpy_dot(self.ctx.h, a, x.h, self.h, z.h, C.byref(out)))
         return out.value
 
     def close(self):

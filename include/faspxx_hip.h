@@ -163,6 +163,8 @@ int fx_dot_dev(fx_ctx ctx, fx_vec x, fx_vec y, double* dev_out);
 int fx_norm2_dev(fx_ctx ctx, fx_vec x, double* dev_out);
 /* Fused: y += a x, then *out = (y, y)^(1/2)  (one pass) */
 int fx_axpy_norm2(fx_ctx ctx, double a, fx_vec x, fx_vec y, double* out);
+/* Fused: y = a y + x (XPAY), then *out = (y, y)^(1/2)  (one pass) */
+int fx_xpay_norm2(fx_ctx ctx, double a, fx_vec x, fx_vec y, double* out);
 /* Fused: y += a x, then *out = (y, z)        (one pass) */
 int fx_axpy_dot(fx_ctx ctx, double a, fx_vec x, fx_vec y, fx_vec z, double* out);
 

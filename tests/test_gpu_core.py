@@ -77,6 +77,30 @@ def test_vec_reductions(ctx, oracle, n):
     assert dx.Dot(dy) == dx.Dot(dy)
 
 
+@pytest.mark.parametrize("n", [1, 4099, 1 << 20])
+def test_vec_fused_updates(ctx, oracle, n):
+    """The fused one-pass forms: the updated vector equals the reference's
+    AXPY / XPAY bit for bit, the reduction matches Norm2 / Dot of it."""
+    rng = np.random.default_rng(n + 3)
+    x, y, z = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(n)
+    a = -0.37
+    d = dev(ctx, y)
+    nrm = d.AXPY_Norm2(a, dev(ctx, x))
+    want = oracle.axpy(a, x, y)
+    assert np.array_equal(d.numpy(), want)
+    assert abs(nrm - oracle.norm2(want)) <= 1e-14 * oracle.norm2(want)
+    d = dev(ctx, y)
+    nrm = d.XPAY_Norm2(a, dev(ctx, x))
+    want = oracle.xpay(a, x, y)
+    assert np.array_equal(d.numpy(), want)
+    assert abs(nrm - oracle.norm2(want)) <= 1e-14 * oracle.norm2(want)
+    d = dev(ctx, y)
+    dt = d.AXPY_Dot(a, dev(ctx, x), dev(ctx, z))
+    want = oracle.axpy(a, x, y)
+    assert np.array_equal(d.numpy(), want)
+    assert abs(dt - oracle.dot(want, z)) <= 1e-13 * np.dot(np.abs(want), np.abs(z))
+
+
 PARITY = [(1, None), (2, 32), (2, None), (3, 256), (4, 16384), (5, 64), (5, None)]
 
 

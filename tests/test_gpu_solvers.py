@@ -344,3 +344,46 @@ def test_ir_mixed_precision(ctx, oracle, problems, cid, scale):
     o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc="jacobi", maxIter=20000, relTol=1e-12,
                      absTol=0.0)
     assert rel_err(xr, o["x"]) < 1e-6
+
+
+@pytest.mark.parametrize("cid,scale", [(3, 128), (1, 64)])
+def test_ir_gmres_inner(ctx, oracle, problems, cid, scale):
+    """FP32-inner GMRES (f64 Hessenberg / Givens) with FP64 refinement, on the
+    nonsymmetric upwind system and on 2D Poisson: the FP64 relative residual
+    meets the outer tolerance and the solution matches the oracle's FP64
+    GMRES to the accuracy the tolerance allows."""
+    pr = problems(cid, scale)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    x = fx.Vec(ctx, pr.A.nrow)
+    res, inner = fx.solve_ir(ctx, A, dev(ctx, pr.b), x, 1.0,
+                             fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
+                             fx.params(maxIter=5000, relTol=1e-4, absTol=0.0), fx.GMRES)
+    assert res.retcode == 0 and res.numIter >= 2 and inner > 0
+    xr = x.numpy()
+    assert np.linalg.norm(pr.b - pr.A.matvec(xr)) < 1e-10 * np.linalg.norm(pr.b)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(GMRES, M, pr.b, np.zeros(M.nrow), pc="jacobi", maxIter=20000,
+                     relTol=1e-12, absTol=0.0)
+    assert rel_err(xr, o["x"]) < 1e-6
+
+
+def test_ir_headline_configuration(ctx, problems):
+    """The benchmark's solve: 3D Poisson 128^3, FP32-inner CG (Jacobi, inner
+    rtol 1e-4) refined in FP64 to 1e-8: two outer steps, FP64 relative
+    residual below 1e-8, about as many inner iterations as the FP64 CG
+    takes, and an error against the exact solution no worse than twice the
+    FP64 CG's."""
+    pr = problems(2, None)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    b = dev(ctx, pr.b)
+    x = fx.Vec(ctx, pr.A.nrow)
+    res, inner = fx.solve_ir(ctx, A, b, x, 1.0, fx.params(maxIter=50, relTol=1e-8, absTol=0.0),
+                             fx.params(maxIter=20000, relTol=1e-4, absTol=0.0))
+    assert res.retcode == 0 and res.numIter == 2
+    xr = x.numpy()
+    assert np.linalg.norm(pr.b - pr.A.matvec(xr)) < 1e-8 * np.linalg.norm(pr.b)
+    r64, x64, _ = gpu_solve(ctx, pr, fx.CG, "jacobi", maxIter=20000, relTol=1e-8, absTol=0.0)
+    assert r64.retcode == 0 and abs(inner - r64.numIter) <= 10
+    # the error against x* (b = A x*) is the FP64 solve's, to within a factor 2
+    e_ir, e_64 = rel_err(xr, pr.xstar), rel_err(x64, pr.xstar)
+    assert e_ir < 2 * e_64 + 1e-12, (e_ir, e_64)
