@@ -1,8 +1,9 @@
 """Benchmark: 3D Poisson 128^3 (7-point, n = 2,097,152, nnz = 14,581,760).
 
 One step = one solve to a relative residual of 1e-8 from x0 = 0 with the
-FP32-inner / FP64-refinement CG (Jacobi preconditioned), inputs resident in
-HBM.  Reported beside it, in the same run: FP64 Jacobi-PCG time-to-solution,
+FP32-inner / FP64-refinement CG (Jacobi preconditioned: the inner CG runs
+on the diagonally scaled matrix S A S, S = diag(a_ii)^-1/2, which is
+Jacobi-PCG in exact arithmetic), inputs resident in HBM.  Reported beside it, in the same run: FP64 Jacobi-PCG time-to-solution,
 standalone SpMV GB/s (f64 and f32, Infinity Cache flushed between reps), 50
 weighted-Jacobi sweeps (omega = 2/3), and the compiled reference CPU path
 (oracle/_ref) solving the same system on one host core.

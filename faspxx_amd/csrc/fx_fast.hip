@@ -10,6 +10,10 @@
 // safe-guard copy) stop the device loop and are run on the host with the
 // per-operation backend, after which the device loop resumes.
 //
+// The refinement's f32 inner solve is CG on the diagonally scaled matrix in
+// a single-reduction form: 2 kernels and 11 vector passes per iteration
+// (see "the scaled inner solver of the refinement" below).
+//
 // CG iteration = 3 kernels (see "the three-kernel iteration" below), replayed
 // as a hipGraph of kChunk iterations; the last iteration of a replay copies
 // the state to a pinned host slot.  Per iteration HBM traffic: the SpMV's
@@ -495,6 +499,253 @@ __global__ void k_add_f32_to_f64(double* __restrict__ x, const float* __restrict
     x[i] = x[i] + (double)d[i];
 }
 
+// ---- the scaled inner solver of the refinement -----------------------------
+// The f32 inner solve runs CG on the symmetrically scaled system
+//   Ahat = S A S,  S = diag(1 / sqrt(a_ii)),  Ahat yhat = S r,  d = S yhat,
+// which is Jacobi-preconditioned CG in exact arithmetic, with no
+// preconditioner vector left in the loop.  It uses the single-reduction
+// form of Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989): with
+// w = Ahat r, gamma = (r, r) and delta = (w, r) from one reduction phase,
+//   beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old),
+//   p = r + beta p,  s = w + beta s  (s = Ahat p),  y += alpha p,  r -= alpha s,
+// so an iteration is two kernels: K1 = SpMV w = Ahat r with partials of
+// (w, r); K2 = the scalars, the four updates and partials of the new (r, r).
+// Per iteration HBM traffic: the matrix + 11 f32 vector passes (K1: r read,
+// w written; K2: r p w s y read, r p s y written), against 13 passes and a
+// third kernel for the f32 Jacobi-PCG.
+struct IrState {
+  double gamma0;      // (r0, r0) of the scaled residual
+  double thr;         // stop when gamma < thr: max(relTol^2 gamma0, absTol^2)
+  double gamma_prev;  // gamma and alpha of the last update
+  double alpha_prev;
+  double gamma, delta;  // the last iteration's sums
+  uint32_t numIter, maxIter;
+  int status, reason;
+};
+enum { IR_CONVERGED = 0, IR_MAXIT = 1, IR_BREAKDOWN = 2 };
+
+// s_i = 1 / sqrt(a_ii); *bad = 1 where row i has no diagonal entry or
+// a_ii is not positive and finite.
+__global__ void k_ir_scale_diag(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                const uint32_t* __restrict__ dp, const double* __restrict__ va,
+                                double* __restrict__ sc, int* __restrict__ bad, uint32_t n) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = dp[i];
+  const bool has = k >= rp[i] && k < rp[i + 1] && ci[k] == i;
+  const double d = has ? va[k] : 0.0;
+  const bool ok = d > 0.0 && d < INFINITY;
+  sc[i] = ok ? 1.0 / sqrt(d) : 0.0;
+  if (!ok) *bad = 1;
+}
+
+// ahat_k = (float)(a_k s_i s_j) for the entries of row i.
+__global__ void k_ir_scale_vals(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                const double* __restrict__ va, const double* __restrict__ sc,
+                                float* __restrict__ vs, uint32_t n) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const double si = sc[i];
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) vs[k] = (float)(va[k] * si * sc[ci[k]]);
+}
+
+// Outer residual for the scaled inner solve: t = b - A x in f64 (Apply +
+// XPAY order), rhat = (float)(s_i t_i); partials of (t, t) and (rhat, rhat).
+struct EpiResidScaled {
+  float* __restrict__ r32;
+  const double* __restrict__ b;
+  const double* __restrict__ sc;
+  __device__ bool skip() const { return false; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
+    const double t = b[i] - v;
+    const float rh = (float)(sc[i] * t);
+    r32[i] = rh;
+    acc[0] = fma(t, t, acc[0]);
+    acc[1] = fma((double)rh, (double)rh, acc[1]);
+  }
+  __device__ void fin(double (&)[2]) const {}
+};
+
+// ||t|| for the host, and the inner solve's starting state.
+struct FinIrInit {
+  double* out;
+  IrState* st;
+  double relTol, absTol;
+  uint32_t maxIter;
+  __device__ bool skip() const { return false; }
+  __device__ void fin(double (&s)[2]) const {
+    if (threadIdx.x != 0) return;
+    out[0] = sqrt(s[0]);
+    IrState S{};
+    S.gamma0 = s[1];
+    S.thr = fmax(relTol * relTol * s[1], absTol * absTol);
+    S.maxIter = maxIter;
+    S.status = ST_RUN;
+    *st = S;
+  }
+};
+
+// K1: w = Ahat r and per-workgroup partials of (w, r).
+struct EpiIrW {
+  float* __restrict__ w;
+  const float* __restrict__ r;
+  const IrState* st;
+  __device__ bool skip() const { return st->status != ST_RUN; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[1]) const {
+    const float wv = (float)v;
+    w[i] = wv;
+    acc[0] = fma((double)wv, (double)r[i], acc[0]);
+  }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+// One pass of K2 over packs c + i * stride, i < kPassPacks (all loaded
+// before use; packs at or past nc are clamped to pack 0 and dropped).
+struct IrPass {
+  Pack16<float> r[kPassPacks], w[kPassPacks], p[kPassPacks], s[kPassPacks], y[kPassPacks];
+  uint32_t c[kPassPacks];
+  bool ok[kPassPacks];
+  __device__ __forceinline__ void load(const float* __restrict__ rv, const float* __restrict__ wv,
+                                       const float* __restrict__ pv, const float* __restrict__ sv,
+                                       const float* __restrict__ yv, uint32_t c0, uint32_t stride,
+                                       uint32_t nc) {
+#pragma unroll
+    for (int i = 0; i < kPassPacks; ++i) {
+      const uint32_t ci = c0 + i * stride;
+      ok[i] = ci < nc;
+      c[i] = ok[i] ? ci : 0u;
+      r[i] = ld16(rv, c[i]);
+      w[i] = ld16(wv, c[i]);
+      p[i] = ld16(pv, c[i]);
+      s[i] = ld16(sv, c[i]);
+      y[i] = ld16(yv, c[i]);
+    }
+  }
+};
+
+// The updates of one element; first: the iteration that starts the
+// recurrences (p = r, s = w, y = alpha p; the old p, s, y are not used).
+__device__ __forceinline__ void ir_elem(bool first, float a, float b, float& ri, float wi,
+                                        float& pi, float& si, float& yi, double& acc) {
+  pi = first ? ri : fma(b, pi, ri);
+  si = first ? wi : fma(b, si, wi);
+  yi = first ? a * pi : fma(a, pi, yi);
+  ri = fma(-a, si, ri);
+  acc = fma((double)ri, (double)ri, acc);
+}
+
+__global__ __launch_bounds__(kWideThreads) void k_ir_update(
+    float* __restrict__ r, const float* __restrict__ w, float* __restrict__ p,
+    float* __restrict__ s, float* __restrict__ y, uint32_t n, IrState* st_cur, IrState* st_next,
+    IrState* hout, const double* __restrict__ p1, unsigned g1, const double* __restrict__ pg_in,
+    double* __restrict__ pg_out) {
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (st_cur->status != ST_RUN) {  // stopped: hand the state on unchanged
+    if (lead) {
+      copy_state(st_next, st_cur);
+      if (hout) copy_state(hout, st_cur);
+    }
+    return;
+  }
+  constexpr int W = Pack16<float>::W;
+  const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
+  const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
+  IrPass pass;  // the first pass is loaded before the scalars are known
+  pass.load(r, w, p, s, y, cfirst, stride, nc);
+  const uint32_t it = st_cur->numIter;
+  double red[2];
+  {
+    double d1[1], d2[1] = {0.0};
+    partials_thread<1, 0u, kWideThreads>(p1, g1, threadIdx.x, d1);
+    if (it > 0) partials_thread<1, 0u, kWideThreads>(pg_in, gridDim.x, threadIdx.x, d2);
+    red[0] = d1[0];
+    red[1] = d2[0];
+  }
+  partials_combine<2, 0u, kWideThreads>(red);
+  const double delta = red[0];
+  const double gamma = it > 0 ? red[1] : st_cur->gamma0;
+  // the same decisions in every workgroup (the same sums of the same partials)
+  int stop = -1;
+  double alpha = 0.0, beta = 0.0;
+  if (gamma < st_cur->thr || gamma == 0.0) {
+    stop = IR_CONVERGED;
+  } else if (it >= st_cur->maxIter) {
+    stop = IR_MAXIT;
+  } else {
+    double den = delta;
+    if (it > 0) {
+      beta = gamma / st_cur->gamma_prev;
+      den = delta - beta * gamma / st_cur->alpha_prev;
+    }
+    alpha = gamma / den;
+    if (!(den > 0.0) || !(fabs(alpha) < INFINITY) || !(fabs(beta) < INFINITY)) stop = IR_BREAKDOWN;
+  }
+  if (stop >= 0) {
+    if (lead) {
+      copy_state(st_next, st_cur);
+      st_next->status = ST_DONE;
+      st_next->reason = stop;
+      st_next->gamma = gamma;
+      st_next->delta = delta;
+      st_cur->reason = stop;
+      st_cur->status = ST_DONE;  // every later iteration of the replay stops
+      if (hout) copy_state(hout, st_next);
+    }
+    return;
+  }
+  const bool first = it == 0;
+  const float a = (float)alpha, b = (float)beta;
+  double acc = 0.0;
+  for (uint32_t c = cfirst;;) {
+#pragma unroll
+    for (int i = 0; i < kPassPacks; ++i) {
+      double ai = 0.0;
+#pragma unroll
+      for (int k = 0; k < W; ++k)
+        ir_elem(first, a, b, pass.r[i].v[k], pass.w[i].v[k], pass.p[i].v[k], pass.s[i].v[k],
+                pass.y[i].v[k], ai);
+      if (pass.ok[i]) {
+        st16(r, pass.c[i], pass.r[i]);
+        st16(p, pass.c[i], pass.p[i]);
+        st16(s, pass.c[i], pass.s[i]);
+        st16(y, pass.c[i], pass.y[i]);
+        acc += ai;
+      }
+    }
+    c += kPassPacks * stride;
+    if (c >= nc) break;
+    pass.load(r, w, p, s, y, c, stride, nc);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
+    const uint32_t i = nc * W + threadIdx.x;
+    float ri = r[i], pi = p[i], si = s[i], yi = y[i];
+    ir_elem(first, a, b, ri, w[i], pi, si, yi, acc);
+    r[i] = ri;
+    p[i] = pi;
+    s[i] = si;
+    y[i] = yi;
+  }
+  double v[1] = {acc};
+  block_partials_n<1, 0u, kWideThreads>(v, pg_out);
+  if (lead) {
+    copy_state(st_next, st_cur);
+    st_next->gamma_prev = gamma;
+    st_next->alpha_prev = alpha;
+    st_next->gamma = gamma;
+    st_next->delta = delta;
+    st_next->numIter = it + 1;
+    if (hout) copy_state(hout, st_next);
+  }
+}
+
+// x += s_i d_i: the scaled correction back in the unscaled unknowns.
+__global__ void k_ir_add_scaled(double* __restrict__ x, const double* __restrict__ sc,
+                                const float* __restrict__ y, uint32_t n) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    x[i] = fma(sc[i], (double)y[i], x[i]);
+}
+
 }  // namespace
 
 CgPrm make_prm(const fx_params& p) {
@@ -916,6 +1167,84 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   return FX_SUCCESS;
 }
 
+// The refinement's scaled inner solve (k_ir_update), replayed in graphs of
+// kChunk iterations on an f32 workspace: r = W.r, w = W.q, p = W.p,
+// s = W.x2, y = W.x.  It starts from the state FinIrInit wrote to state
+// buffer 0 and r from EpiResidScaled.
+struct IrInner {
+  FastWs& W;
+  fx_csr As;  // A's structure with the scaled f32 values
+  hipStream_t st;
+
+  IrState* dstate(int c) const { return (IrState*)W.state + (c & 1); }
+  IrState* hslot(int slot) const { return (IrState*)W.hstate + slot; }
+  // partials of (r, r): written by iteration c into buffer (c + 1) & 1
+  double* pg(int c) const { return W.partials + W.part2_off + (size_t)(c & 1) * kMaxRedBlocks; }
+
+  int launch_iteration(int c, IrState* hout) {
+    float* r = (float*)W.r;
+    float* w = (float*)W.q;
+    FX_TRY((launch_spmv<float, float, ROW_AX, 1, 0u>(As, st, XLoad<float>{r}, BNone{},
+                                                     EpiIrW{w, r, dstate(c)}, W.partials,
+                                                     nullptr, nullptr, false)));
+    const unsigned g1 = spmv_grid(As->nblk, W.ctx->sms);
+    const unsigned g = wide_blocks(W.n, Pack16<float>::W, W.ctx->sms, kPassPacks);
+    hipLaunchKernelGGL(k_ir_update, dim3(g), dim3(kWideThreads), 0, st, r, (const float*)w,
+                       (float*)W.p, (float*)W.x2, (float*)W.x, W.n, dstate(c), dstate(c + 1),
+                       hout, (const double*)W.partials, g1, (const double*)pg(c), pg(c + 1));
+    FX_HIP(hipGetLastError());
+    return 0;
+  }
+
+  int build_graph() {
+    if (W.chunk[0]) return 0;
+    for (int slot = 0; slot < 2; ++slot) {
+      hipGraph_t g = nullptr;
+      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int c = 0; c < kChunk && !rc; ++c)
+        rc = launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      FX_HIP(e);
+      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      FX_HIP(e);
+    }
+    return 0;
+  }
+
+  int launch_chunk(int slot) {
+    FX_HIP(hipGraphLaunch(W.chunk[slot], st));
+    FX_HIP(hipEventRecord(W.ev[slot], st));
+    return 0;
+  }
+
+  // Replays until the device loop stops; out = the final state.  A replay
+  // that finds the loop running advances it by kChunk iterations, so it
+  // stops within maxIter / kChunk + 1 replays.
+  int run(uint32_t maxIter, IrState& out) {
+    FX_TRY(build_graph());
+    const uint64_t max_chunks = (uint64_t)maxIter / kChunk + 4;
+    FX_TRY(launch_chunk(0));
+    FX_TRY(launch_chunk(1));
+    uint64_t chunks = 2;
+    int cur = 0;
+    for (;;) {
+      FX_HIP(hipEventSynchronize(W.ev[cur]));
+      const IrState* hs = hslot(cur);
+      if (hs->status != ST_RUN) {
+        out = *hs;
+        return 0;
+      }
+      FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "IR inner loop did not stop");
+      FX_TRY(launch_chunk(cur));
+      ++chunks;
+      cur ^= 1;
+    }
+  }
+};
+
 }  // namespace
 
 bool fast_eligible(fx_solver S) {
@@ -982,6 +1311,13 @@ struct fx_ir_s {
   double* dsc = nullptr;
   double* part = nullptr;
   FastWs* W = nullptr;
+  // scaled inner solve (CG inner, positive diagonal): s_i = 1 / sqrt(a_ii),
+  // the f32 values of S A S, and a matrix handle that shares A's structure
+  bool scaled = false;
+  double* sc = nullptr;
+  float* vs32 = nullptr;
+  int* bad = nullptr;
+  fx_csr_s As{};
   ~fx_ir_s() {
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
     delete W;
@@ -991,8 +1327,54 @@ struct fx_ir_s {
     (void)hipFree(d32);
     (void)hipFree(dsc);
     (void)hipFree(part);
+    (void)hipFree(sc);
+    (void)hipFree(vs32);
+    (void)hipFree(bad);
   }
 };
+
+namespace {
+
+bool ir_refcg_forced() {
+  const char* e = std::getenv("FASPXX_FX_IR_REFCG");
+  return e && e[0] == '1';
+}
+
+// Form S and S A S for the scaled inner solve.  Returns 0 with ir->scaled
+// set when every diagonal entry is present, positive and finite, 0 with
+// ir->scaled clear otherwise, or an error code.
+int ir_scale(fx_ir ir) {
+  fx_csr A = ir->A;
+  const uint32_t n = A->nrow;
+  hipStream_t st = ir->ctx->stream;
+  FX_HIP(hipMalloc(&ir->sc, sizeof(double) * pack_round(n)));
+  FX_HIP(hipMalloc(&ir->vs32, sizeof(float) * (A->nnz ? A->nnz : 1)));
+  FX_HIP(hipMalloc(&ir->bad, sizeof(int)));
+  FX_HIP(hipMemsetAsync(ir->bad, 0, sizeof(int), st));
+  const unsigned g = (n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_ir_scale_diag, dim3(g), dim3(kThreads), 0, st, A->rp, A->ci, A->dp, A->va,
+                     ir->sc, ir->bad, n);
+  FX_HIP(hipGetLastError());
+  int bad = 0;
+  FX_HIP(hipMemcpyAsync(&bad, ir->bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  FX_HIP(hipStreamSynchronize(st));
+  if (bad) {
+    (void)hipFree(ir->sc);
+    (void)hipFree(ir->vs32);
+    ir->sc = nullptr;
+    ir->vs32 = nullptr;
+    return 0;
+  }
+  hipLaunchKernelGGL(k_ir_scale_vals, dim3(g), dim3(kThreads), 0, st, A->rp, A->ci, A->va,
+                     ir->sc, ir->vs32, n);
+  FX_HIP(hipGetLastError());
+  ir->As = *A;  // the same device arrays; only va32 differs, and nothing frees through As
+  ir->As.va32 = ir->vs32;
+  ir->scaled = true;
+  return 0;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1003,7 +1385,6 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
   FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "IR needs a square matrix");
   FX_CHECK(inner_type == FX_SOLVER_CG || inner_type == FX_SOLVER_GMRES, FX_ERROR_SOLVER_TYPE,
            "IR inner solver must be CG or GMRES");
-  if (!A->va32 && A->has_values) FX_TRY(fx_csr_make_f32(A));
   fx_ir ir = new fx_ir_s();
   ir->ctx = ctx;
   ir->inner_type = inner_type;
@@ -1015,11 +1396,24 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
     delete ir;
     return rc;
   };
-  int rc = fx_pc_create_jacobi(ctx, A, weight, 1, &ir->pc);
-  if (!rc) rc = ensure_dinv32(ir->pc);
-  if (!rc) rc = fx_solver_create(ctx, inner_type, &ir->inner_s);
-  if (!rc) rc = fx_solver_setup(ir->inner_s, A, ir->pc, inner);
-  if (rc) return fail(rc);
+  int rc = 0;
+  if (inner_type == FX_SOLVER_CG && A->has_values && n > 0 && !ir_refcg_forced()) {
+    try {
+      rc = ir_scale(ir);
+    } catch (const OpsError& e) {
+      rc = e.code;
+    }
+    if (rc) return fail(rc);
+  }
+  if (!ir->scaled) {
+    // f32 copy of A for the Jacobi-preconditioned inner solver
+    if (!A->va32 && A->has_values) rc = fx_csr_make_f32(A);
+    if (!rc) rc = fx_pc_create_jacobi(ctx, A, weight, 1, &ir->pc);
+    if (!rc) rc = ensure_dinv32(ir->pc);
+    if (!rc) rc = fx_solver_create(ctx, inner_type, &ir->inner_s);
+    if (!rc) rc = fx_solver_setup(ir->inner_s, A, ir->pc, inner);
+    if (rc) return fail(rc);
+  }
   const size_t nb = n ? n : 1;
   const size_t np = 8 * ((size_t)spmv_grid(A->nblk, ctx->sms) + kMaxRedBlocks);
   if (hipMalloc(&ir->r32, sizeof(float) * nb) || hipMalloc(&ir->d32, sizeof(float) * nb) ||
@@ -1030,7 +1424,8 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
   }
   if (inner_type == FX_SOLVER_CG) {
     ir->W = new FastWs();
-    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sizeof(CgState), spmv_grid(A->nblk, ctx->sms));
+    const size_t sb = sizeof(CgState) > sizeof(IrState) ? sizeof(CgState) : sizeof(IrState);
+    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sb, spmv_grid(A->nblk, ctx->sms));
     if (rc) return fail(rc);
   }
   *out = ir;
@@ -1057,9 +1452,21 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
   try {
     for (;;) {
       // r = b - A x (f64 values, Apply + XPAY order), r32 = (float) r, ||r||
-      rc = launch_spmv<double, double, ROW_AX, 1, 0u>(
-          A, st, XLoad<double>{(const double*)x->d}, BNone{},
-          EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part);
+      // (scaled: W.r = (float)(S r), and the inner solve's starting state)
+      if (ir->scaled) {
+        rc = launch_spmv<double, double, ROW_AX, 2, 0u>(
+            A, st, XLoad<double>{(const double*)x->d}, BNone{},
+            EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc}, ir->part, nullptr,
+            nullptr, false);
+        if (!rc)
+          rc = launch_finish<2, 0u>(st, ir->part, spmv_grid(A->nblk, ctx->sms),
+                                    FinIrInit{ir->dsc, (IrState*)ir->W->state, ir->inner.relTol,
+                                              ir->inner.absTol, ir->inner.maxIter});
+      } else {
+        rc = launch_spmv<double, double, ROW_AX, 1, 0u>(
+            A, st, XLoad<double>{(const double*)x->d}, BNone{},
+            EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part);
+      }
       if (rc) break;
       double h = 0;
       if (hipMemcpyAsync(&h, ir->dsc, sizeof(double), hipMemcpyDeviceToHost, st) ||
@@ -1075,6 +1482,23 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         break;
       }
       ++outer_it;
+      if (ir->scaled) {
+        // S A S y = S r in f32 from y = 0; x += S y
+        IrInner in{*ir->W, &ir->As, st};
+        IrState fs{};
+        rc = in.run(ir->inner.maxIter, fs);
+        if (rc) break;
+        inner_total += fs.numIter;
+        if (fs.numIter > 0) {
+          hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                             (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x, n);
+          if (hipGetLastError() != hipSuccess) {
+            rc = FX_ERROR_DEVICE;
+            break;
+          }
+        }
+        continue;
+      }
       // inner solve A d = r in f32 from d = 0
       if (hipMemsetAsync(ir->d32, 0, sizeof(float) * n, st)) {
         rc = FX_ERROR_DEVICE;

@@ -312,13 +312,22 @@ int fx_krylov(fx_ctx ctx, fx_csr A, fx_vec b, fx_vec x, fx_pc pc, const char* al
  * inner solve A d = r in f32 values and vectors by `inner_type` (CG, or
  * GMRES with f64 Hessenberg / Givens) with inner->relTol, from d = 0,
  * Jacobi(weight) preconditioned; x += d in f64.  result->numIter is the
- * number of outer steps; *inner_iters (optional) the summed inner count. */
+ * number of outer steps; *inner_iters (optional) the summed inner count.
+ * CG inner, every a_ii > 0: the inner solve is CG on S A S with
+ * S = diag(a_ii)^-1/2 (Jacobi-PCG in exact arithmetic; the weight does not
+ * change CG's iterates), in the single-reduction form of Chronopoulos and
+ * Gear, stopped when ||S r_k|| < max(inner->relTol ||S r_0||,
+ * inner->absTol) or after inner->maxIter iterations; then d = S y.  Other
+ * matrices, or FASPXX_FX_IR_REFCG=1 at creation, use the f32 Jacobi-PCG
+ * with the reference CG's checks. */
 int fx_solve_ir(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_params* outer,
                 const fx_params* inner, fx_vec b, fx_vec x, fx_result* result,
                 uint32_t* inner_iters);
-/* The same as a reusable object: create allocates the workspace (f32 value
- * copy, Jacobi diagonal, inner-solver buffers); solve allocates nothing
- * except on its first call, which also builds the replay graph. */
+/* The same as a reusable object: create allocates the workspace (the f32
+ * values of S A S and S, or an f32 value copy and the Jacobi diagonal;
+ * inner-solver buffers); solve allocates nothing except on its first call,
+ * which also builds the replay graphs.  Values of A changed after creation
+ * are not seen. */
 int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_params* outer,
                  const fx_params* inner, fx_ir* out);
 int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner_iters);

@@ -3,6 +3,7 @@ src/BiCGStab.cxx, src/GMRES.cxx, src/FGMRES.cxx, src/Iter.cxx), through the
 C ABI.  Bars: iteration counts within +-2 at the same relTol; solutions and
 final residual norms within the stated relative tolerances; weighted-Jacobi
 sweeps bit-exact (their updates contain no reduction)."""
+import dataclasses
 import json
 import os
 import pathlib
@@ -344,6 +345,72 @@ def test_ir_mixed_precision(ctx, oracle, problems, cid, scale):
     o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc="jacobi", maxIter=20000, relTol=1e-12,
                      absTol=0.0)
     assert rel_err(xr, o["x"]) < 1e-6
+
+
+def _ir_solve(ctx, pr, outer, inner, monkeypatch, refcg):
+    """One refinement solve; refcg forces the f32 Jacobi-PCG inner solver
+    instead of the scaled single-reduction one (read when the solver is made)."""
+    if refcg:
+        monkeypatch.setenv("FASPXX_FX_IR_REFCG", "1")
+    else:
+        monkeypatch.delenv("FASPXX_FX_IR_REFCG", raising=False)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    x = fx.Vec(ctx, pr.A.nrow)
+    ir = fx.IR(ctx, A, 1.0, outer, inner)
+    res, inner_its = ir.Solve(dev(ctx, pr.b), x)
+    ir.close()
+    monkeypatch.delenv("FASPXX_FX_IR_REFCG", raising=False)
+    return res, inner_its, x.numpy()
+
+
+@pytest.mark.parametrize("cid,scale", [(2, 32), (5, 64), (4, 16384), (1, 64)])
+def test_ir_scaled_inner_matches_jacobi_inner(ctx, problems, monkeypatch, cid, scale):
+    """The scaled inner solver (CG on S A S, S = diag(a_ii)^-1/2, one
+    reduction phase per iteration) is Jacobi-PCG in exact arithmetic: with
+    the same tolerances it reaches the FP64 outer tolerance in the same
+    number of outer steps (+-1) and a similar number of inner iterations as
+    the f32 Jacobi-PCG inner solver, and the two solutions agree to the
+    accuracy the tolerance allows."""
+    pr = problems(cid, scale)
+    outer = fx.params(maxIter=50, relTol=1e-10, absTol=0.0)
+    inner = fx.params(maxIter=5000, relTol=1e-4, absTol=0.0)
+    rs, its_s, xs = _ir_solve(ctx, pr, outer, inner, monkeypatch, False)
+    rj, its_j, xj = _ir_solve(ctx, pr, outer, inner, monkeypatch, True)
+    assert rs.retcode == 0 and rj.retcode == 0
+    nb = np.linalg.norm(pr.b)
+    assert np.linalg.norm(pr.b - pr.A.matvec(xs)) < 1e-10 * nb
+    assert abs(rs.numIter - rj.numIter) <= 1
+    assert abs(its_s - its_j) <= max(4, 0.15 * its_j), (its_s, its_j)
+    assert rel_err(xs, xj) < 1e-8
+
+
+def test_ir_nonpositive_diagonal_uses_jacobi_inner(ctx, problems, monkeypatch):
+    """A matrix with a non-positive diagonal (here -A of 2D Poisson) cannot
+    be scaled by diag(a_ii)^-1/2: the refinement then runs the f32
+    Jacobi-PCG inner solver, bit for bit the same solve as when that solver
+    is forced, and still converges (the signs cancel in Jacobi-PCG)."""
+    pr = problems(1, 32)
+    negA = dataclasses.replace(pr.A, values=-pr.A.values)
+    neg = dataclasses.replace(pr, A=negA, b=-pr.b)
+    outer = fx.params(maxIter=50, relTol=1e-10, absTol=0.0)
+    inner = fx.params(maxIter=5000, relTol=1e-4, absTol=0.0)
+    rd, its_d, xd = _ir_solve(ctx, neg, outer, inner, monkeypatch, False)
+    rf, its_f, xf = _ir_solve(ctx, neg, outer, inner, monkeypatch, True)
+    assert rd.retcode == 0 and its_d == its_f and rd.numIter == rf.numIter
+    assert np.array_equal(xd, xf)
+    assert np.linalg.norm(neg.b - neg.A.matvec(xd)) < 1e-10 * np.linalg.norm(neg.b)
+
+
+def test_ir_scaled_inner_iteration_cap(ctx, problems, monkeypatch):
+    """An inner solve cut off by its maxIter still hands its iterate to the
+    outer loop, which then converges in more outer steps."""
+    pr = problems(1, 16)
+    outer = fx.params(maxIter=400, relTol=1e-10, absTol=0.0)
+    inner = fx.params(maxIter=10, relTol=1e-6, absTol=0.0)
+    res, its, xs = _ir_solve(ctx, pr, outer, inner, monkeypatch, False)
+    assert res.retcode == 0 and res.numIter > 2
+    assert its <= 10 * res.numIter
+    assert np.linalg.norm(pr.b - pr.A.matvec(xs)) < 1e-10 * np.linalg.norm(pr.b)
 
 
 @pytest.mark.parametrize("cid,scale", [(3, 128), (1, 64)])
