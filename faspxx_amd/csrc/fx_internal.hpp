@@ -130,9 +130,12 @@ struct fx_pc_s {
 namespace fx {
 
 // Row-block geometry of the SpMV kernels (LDS stage of kBlockNnz entries).
+// 1792 entries = 256 rows of 7: with 16-bit columns an f64 block takes
+// 19 KB of LDS, so eight blocks fit in a CU's 160 KB.
 #ifndef FX_BLOCK_NNZ
-#define FX_BLOCK_NNZ 2048
+#define FX_BLOCK_NNZ 1792
 #endif
+static_assert(FX_BLOCK_NNZ % 256 == 0, "a whole number of staged entries per thread");
 #ifndef FX_BLOCK_ROWS
 #define FX_BLOCK_ROWS 256
 #endif

@@ -23,6 +23,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "fx_internal.hpp"
 
 namespace fx {
@@ -53,14 +55,6 @@ inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
   return {A->rp, A->rpb, A->ci, A->cd, A->has_values ? A->va32 : nullptr, A->blk, A->nblk,
           A->nnz};
 }
-
-// LDS footprint: row pointers + staged (column, value) pairs.
-template <class MT, class VT>
-struct SpmvLds {
-  uint32_t rp[kBlockRows + 1];
-  uint32_t ci[kBlockNnz];
-  MT va[kBlockNnz];
-};
 
 // Threads per row for a block of nr rows holding nz entries: one thread per
 // row (the reference's summation order) while rows average at most
@@ -101,6 +95,23 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
 #ifndef FX_RP16
 #define FX_RP16 1
 #endif
+
+// LDS image of a row block: row pointers relative to the block's first
+// entry, then the staged (column, value) pairs.  CD: the columns stay in
+// LDS as the 16-bit deltas to the block's first row r0 that were loaded,
+// 10 bytes per f64 entry instead of 12, so that more blocks fit in a CU.
+template <class MT, bool CD>
+struct SpmvLds {
+  using RpT = std::conditional_t<FX_RP16 != 0, uint16_t, uint32_t>;
+  using CiT = std::conditional_t<CD, int16_t, uint32_t>;
+  RpT rp[kBlockRows + 1];
+  CiT ci[kBlockNnz];
+  MT va[kBlockNnz];
+  __device__ __forceinline__ uint32_t col(uint32_t k, uint32_t r0) const {
+    if constexpr (CD) return (uint32_t)((int32_t)r0 + (int32_t)ci[k]);
+    else return ci[k];
+  }
+};
 template <class T>
 __device__ __forceinline__ T stream_load(const T* p) {
   if constexpr (FX_NT_MATRIX) return __builtin_nontemporal_load(p);
@@ -114,20 +125,20 @@ __device__ __forceinline__ T stream_load(const T* p) {
 // after them, so all of a thread's loads are in flight before the first
 // wait.  CD: 16-bit column deltas to r0; VA: the matrix has values.
 template <bool CD, bool VA, class MT, class VT>
-__device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, VT>& lds,
+__device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CD>& lds,
                                             uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
                                             int tid) {
   constexpr int kPer = kBlockNnz / kThreads;
   // last entry index a load may touch: the block's last entry, or for a
   // block of empty rows the matrix's last entry (nothing is stored then)
   const uint32_t klast = nz ? k0 + nz - 1 : in.nnz - 1;
-  uint32_t cix[kPer];
+  typename SpmvLds<MT, CD>::CiT cix[kPer];
   MT vv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
     if constexpr (CD)
-      cix[j] = (uint32_t)(int32_t)stream_load(&in.cd[k]);
+      cix[j] = stream_load(&in.cd[k]);
     else
       cix[j] = stream_load(&in.ci[k]);
     if constexpr (VA) vv[j] = stream_load(&in.va[k]);
@@ -154,7 +165,7 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = tid + j * kThreads;
     if (k < nz) {
-      lds.ci[k] = CD ? (uint32_t)((int32_t)r0 + (int32_t)cix[j]) : cix[j];
+      lds.ci[k] = cix[j];
       lds.va[k] = VA ? vv[j] : (MT)1;
     }
   }
@@ -178,7 +189,8 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
 //   bool skip() const                      uniform early exit (solver stop flag)
 //   void row(uint32_t r, double v, double (&acc)[NRED])
 //   void fin(double (&sum)[NRED])          run by k_finish after the kernel
-template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
+template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi,
+          bool CD>
 __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF bf, Epi epi0,
                                                    double* partials) {
   if (epi0.skip()) return;
@@ -186,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
   Epi epi = epi0;
   maybe_init(xf);
   maybe_init(epi);
-  __shared__ SpmvLds<MT, VT> lds;
+  __shared__ SpmvLds<MT, CD> lds;
   double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
   for (int k = 0; k < (NRED > 0 ? NRED : 1); ++k) acc[k] = 0.0;
@@ -207,7 +219,10 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         for (int j = 0; j < 4; ++j) {
           const uint32_t k = min(kb + j * kThreads, k1 - 1);
           a[j] = in.va ? (double)in.va[k] : 1.0;
-          c[j] = in.cd ? (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]) : in.ci[k];
+          if constexpr (CD)
+            c[j] = (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]);
+          else
+            c[j] = in.ci[k];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
@@ -222,12 +237,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         epi.row(r0, v, acc);
       }
     } else {
-      if (in.cd)
-        in.va ? stage_block<true, true>(in, lds, r0, k0, nr, nz, tid)
-              : stage_block<true, false>(in, lds, r0, k0, nr, nz, tid);
-      else
-        in.va ? stage_block<false, true>(in, lds, r0, k0, nr, nz, tid)
-              : stage_block<false, false>(in, lds, r0, k0, nr, nz, tid);
+      in.va ? stage_block<CD, true>(in, lds, r0, k0, nr, nz, tid)
+            : stage_block<CD, false>(in, lds, r0, k0, nr, nz, tid);
       __syncthreads();
       const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {
@@ -263,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
                 for (int j = 0; j < 4; ++j) {
                   const uint32_t k = min(kb + j * kThreads, e - 1);
                   a[j] = (double)lds.va[k];
-                  c[j] = lds.ci[k];
+                  c[j] = lds.col(k, r0);
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
               const uint32_t k = min(base + j, e - 1);
-              cj[j] = lds.ci[k];
+              cj[j] = lds.col(k, r0);
               av[j] = (double)lds.va[k];
             }
 #pragma unroll
@@ -324,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
             for (int j = 0; j < 4; ++j) {
               const uint32_t k = min(kb + j * tpr, e - 1);
               a[j] = (double)lds.va[k];
-              c[j] = lds.ci[k];
+              c[j] = lds.col(k, r0);
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
@@ -376,17 +387,31 @@ struct EpiStore {
 
 // With events e0 / e1 the kernel is launched with them attached to its
 // dispatch (hipExtLaunchKernelGGL), so that they time the kernel alone.
+// The kernel is instantiated for 16-bit column deltas (CD) and for u32
+// columns; A->cd picks one.
+template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi,
+          bool CD>
+inline void launch_spmv_k(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
+                          hipEvent_t e0, hipEvent_t e1, unsigned g) {
+  if (e0)
+    hipExtLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CD>), dim3(g),
+                          dim3(kThreads), 0, st, e0, e1, 0, spmv_in<MT, VT>(A), xf, bf, epi,
+                          partials);
+  else
+    hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CD>), dim3(g),
+                       dim3(kThreads), 0, st, spmv_in<MT, VT>(A), xf, bf, epi, partials);
+}
+
 template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
 inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finisher = true) {
   const unsigned g = spmv_grid(A->nblk, A->ctx->sms);
-  if (e0)
-    hipExtLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g),
-                          dim3(kThreads), 0, st, e0, e1, 0, spmv_in<MT, VT>(A), xf, bf, epi,
-                          partials);
+  if (A->cd)
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, true>(A, st, xf, bf, epi, partials,
+                                                                  e0, e1, g);
   else
-    hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi>), dim3(g), dim3(kThreads),
-                       0, st, spmv_in<MT, VT>(A), xf, bf, epi, partials);
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, false>(A, st, xf, bf, epi, partials,
+                                                                   e0, e1, g);
   FX_HIP(hipGetLastError());
   if constexpr (NRED > 0) {
     if (finisher) return launch_finish<NRED, MAXMASK>(st, partials, g, epi);
