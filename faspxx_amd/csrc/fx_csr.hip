@@ -1,8 +1,11 @@
 // fx_csr.hip -- CSRx matrices in HBM: creation, SpMV (MAT::Apply),
 // residual (MAT::Residual), diagonal (MAT::GetDiag) and transpose
 // (MAT::TransInPlace), src/MAT.cxx.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <unordered_map>
+#include <vector>
 
 #include "fx_spmv.hpp"
 
@@ -218,7 +221,59 @@ int csr_alloc(fx_csr A) {
   return 0;
 }
 
-int csr_set_blocks(fx_csr A, const uint32_t* host_rp) {
+// Row patterns of a matrix given on the host: pid[i] and a dictionary of
+// plen-strided offset lists (col - row).  False when some row is longer than
+// kPatMaxLen, there are more than kPatMax distinct patterns, or the
+// dictionary exceeds kPatDict offsets.  Each
+// pattern is found by a hash of (length, offsets) and then compared entry by
+// entry, so a hash collision cannot merge two patterns.
+bool build_patterns(uint32_t nrow, const uint32_t* rp, const uint32_t* ci, uint32_t max_row,
+                    std::vector<uint8_t>& pid, std::vector<int32_t>& dict, uint32_t& plen,
+                    uint32_t& npat) {
+  if (max_row > kPatMaxLen || nrow == 0) return false;
+  plen = max_row ? max_row : 1;
+  std::vector<std::vector<int32_t>> pats;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> byhash;
+  pid.assign(nrow, 0);
+  std::vector<int32_t> off(kPatMaxLen);
+  for (uint32_t i = 0; i < nrow; ++i) {
+    const uint32_t len = rp[i + 1] - rp[i];
+    uint64_t h = 1469598103934665603ull ^ len;  // FNV-1a over the offsets
+    for (uint32_t j = 0; j < len; ++j) {
+      off[j] = (int32_t)((int64_t)ci[rp[i] + j] - (int64_t)i);
+      h = (h ^ (uint32_t)off[j]) * 1099511628211ull;
+    }
+    auto& ids = byhash[h];
+    int found = -1;
+    for (uint32_t id : ids) {
+      const auto& q = pats[id];
+      if (q.size() == len && std::equal(q.begin(), q.end(), off.begin())) {
+        found = (int)id;
+        break;
+      }
+    }
+    if (found < 0) {
+      if (pats.size() >= kPatMax) return false;
+      found = (int)pats.size();
+      pats.emplace_back(off.begin(), off.begin() + len);
+      ids.push_back((uint32_t)found);
+    }
+    pid[i] = (uint8_t)found;
+  }
+  npat = (uint32_t)pats.size();
+  if ((size_t)npat * plen > kPatDict) return false;
+  dict.assign((size_t)npat * plen, 0);
+  for (uint32_t p = 0; p < npat; ++p)
+    std::copy(pats[p].begin(), pats[p].end(), dict.begin() + (size_t)p * plen);
+  return true;
+}
+
+bool patterns_disabled() {
+  const char* e = std::getenv("FASPXX_FX_NOPAT");
+  return e && e[0] == '1';
+}
+
+int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
   uint32_t mx = 0;
   const std::vector<uint32_t> rows = build_row_blocks(A->nrow, host_rp, &mx);
   A->nblk = (uint32_t)rows.size() - 1;
@@ -239,9 +294,27 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp) {
   FX_HIP(hipMalloc(&A->rpb, sizeof(uint16_t) * rpb.size()));
   FX_HIP(hipMemcpyAsync(A->rpb, rpb.data(), sizeof(uint16_t) * rpb.size(), hipMemcpyHostToDevice,
                         st));
-  // 2-byte column deltas when every block admits them (FASPXX_FX_COL32=1 keeps u32 columns)
+  // 2-byte column deltas when every block admits them (FASPXX_FX_COL32=1 keeps u32 columns);
+  // before them, row patterns for a matrix given on the host (FASPXX_FX_NOPAT=1: none)
   const char* e = std::getenv("FASPXX_FX_COL32");
-  if (A->nnz && A->nblk && !(e && e[0] == '1')) {
+  const bool col32 = e && e[0] == '1';
+  if (host_ci && A->nnz && A->nblk && !col32 && !patterns_disabled() && kBlockRows == kThreads) {
+    std::vector<uint8_t> pid;
+    std::vector<int32_t> dict;
+    uint32_t plen = 0, npat = 0;
+    if (build_patterns(A->nrow, host_rp, host_ci, mx, pid, dict, plen, npat)) {
+      FX_HIP(hipMalloc(&A->pid, pid.size()));
+      FX_HIP(hipMalloc(&A->pdict, sizeof(int32_t) * dict.size()));
+      FX_HIP(hipMemcpyAsync(A->pid, pid.data(), pid.size(), hipMemcpyHostToDevice, st));
+      FX_HIP(hipMemcpyAsync(A->pdict, dict.data(), sizeof(int32_t) * dict.size(),
+                            hipMemcpyHostToDevice, st));
+      A->plen = plen;
+      A->npat = npat;
+      FX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+      return 0;
+    }
+  }
+  if (A->nnz && A->nblk && !col32) {
     uint32_t* bad = nullptr;
     FX_HIP(hipMalloc(&A->cd, sizeof(int16_t) * A->nnz));
     FX_HIP(hipMalloc(&bad, sizeof(uint32_t)));
@@ -276,6 +349,8 @@ void csr_free(fx_csr A) {
   (void)hipFree(A->blk);
   (void)hipFree(A->cd);
   (void)hipFree(A->rpb);
+  (void)hipFree(A->pid);
+  (void)hipFree(A->pdict);
   (void)hipFree(A->src);
   delete A;
 }
@@ -303,7 +378,7 @@ int csr_finish_device(fx_csr A) {
   FX_HIP(hipMemcpyAsync(hrp.data(), A->rp, sizeof(uint32_t) * hrp.size(), hipMemcpyDeviceToHost,
                         st));
   FX_HIP(hipStreamSynchronize(st));
-  return csr_set_blocks(A, hrp.data());
+  return csr_set_blocks(A, hrp.data(), nullptr);
 }
 
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype, hipEvent_t e0,
@@ -396,7 +471,7 @@ int fx_csr_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, uint32_t nnz, const 
     hipLaunchKernelGGL(k_form_diagptr, dim3((nrow + kThreads - 1) / kThreads), dim3(kThreads), 0,
                        st, A->rp, A->ci, A->dp, nrow);
   }
-  rc = csr_set_blocks(A, rowPtr);
+  rc = csr_set_blocks(A, rowPtr, colInd);
   if (rc) return fail(rc);
   *out = A;
   return 0;
@@ -410,7 +485,7 @@ int fx_csr_destroy(fx_csr A) {
 int fx_csr_index_bytes(fx_csr A, int* bytes) {
   clear_error();
   FX_CHECK(A && bytes, FX_ERROR_INPUT_PAR, "null argument");
-  *bytes = A->cd ? 2 : 4;
+  *bytes = A->pid ? 0 : A->cd ? 2 : 4;
   return 0;
 }
 
@@ -610,7 +685,7 @@ int fx_csr_transpose(fx_ctx ctx, fx_csr A, fx_csr* At) {
   }
   keep_sorted();
   cleanup();
-  rc = csr_set_blocks(T, hrp.data());
+  rc = csr_set_blocks(T, hrp.data(), nullptr);
   if (rc) {
     csr_free(T);
     return rc;

@@ -112,6 +112,14 @@ struct fx_csr_s {
   // (a block holds at most kBlockNnz entries or one row): what the SpMV
   // kernels read in place of rp.
   uint16_t* rpb = nullptr;
+  // Row patterns, in place of cd / ci in the SpMV kernels, for matrices made
+  // from host arrays whose rows repeat at most 255 column-offset patterns of
+  // at most kPatMaxLen entries (stencil matrices): row i's entry j is in
+  // column i + pdict[pid[i] * plen + j].  1 byte per row instead of 2 per
+  // entry.
+  uint8_t* pid = nullptr;
+  int32_t* pdict = nullptr;
+  uint32_t plen = 0, npat = 0;
   // set on a transpose: entry p of this matrix is entry src[p] of its source
   uint32_t* src = nullptr;
 };
@@ -142,6 +150,9 @@ static_assert(FX_BLOCK_NNZ % 256 == 0, "a whole number of staged entries per thr
 constexpr uint32_t kBlockRows = FX_BLOCK_ROWS;  // a multiple of kThreads
 static_assert(kBlockRows % kThreads == 0, "row blocks hold whole passes of the workgroup");
 constexpr uint32_t kBlockNnz = FX_BLOCK_NNZ;
+constexpr uint32_t kPatMaxLen = 32;  // longest row the row-pattern format takes
+constexpr uint32_t kPatMax = 255;    // most patterns it takes (1-byte ids)
+constexpr uint32_t kPatDict = 1024;  // largest dictionary (patterns x plen offsets) it takes
 
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

@@ -41,19 +41,30 @@ struct SpmvIn {
   const uint32_t* __restrict__ blk;  // (first row, first entry) of each block, nblk + 1 pairs
   uint32_t nblk;
   uint32_t nnz;  // >= 1 whenever a kernel runs
+  const uint8_t* __restrict__ pid;     // row patterns: pattern of each row, or null
+  const int32_t* __restrict__ pdict;   // offsets col - row, plen per pattern
+  uint32_t plen;
+  uint32_t dlen;  // npat * plen
 };
+
+// Column formats of the staged entries: u32 columns, 16-bit deltas to the
+// block's first row, or row patterns (no per-entry column data).
+enum { CF_U32 = 0, CF_D16 = 1, CF_PAT = 2 };
 
 template <class MT, class VT>
 inline SpmvIn<MT, VT> spmv_in(fx_csr A);
 
 template <>
 inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
-  return {A->rp, A->rpb, A->ci, A->cd, A->has_values ? A->va : nullptr, A->blk, A->nblk, A->nnz};
+  return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va : nullptr,
+          A->blk,  A->nblk, A->nnz, A->pid,   A->pdict,
+          A->plen, A->npat * A->plen};
 }
 template <>
 inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
-  return {A->rp, A->rpb, A->ci, A->cd, A->has_values ? A->va32 : nullptr, A->blk, A->nblk,
-          A->nnz};
+  return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va32 : nullptr,
+          A->blk,  A->nblk, A->nnz, A->pid,   A->pdict,
+          A->plen, A->npat * A->plen};
 }
 
 // Threads per row for a block of nr rows holding nz entries: one thread per
@@ -97,18 +108,23 @@ __device__ __forceinline__ int threads_per_row(uint32_t nr, uint32_t nz) {
 #endif
 
 // LDS image of a row block: row pointers relative to the block's first
-// entry, then the staged (column, value) pairs.  CD: the columns stay in
+// entry, then the staged (column, value) pairs.  CF_D16: the columns stay in
 // LDS as the 16-bit deltas to the block's first row r0 that were loaded,
 // 10 bytes per f64 entry instead of 12, so that more blocks fit in a CU.
-template <class MT, bool CD>
+// CF_PAT: values, and a copy of the pattern dictionary (columns come from
+// the row's pattern).
+template <class MT, int CF>
 struct SpmvLds {
   using RpT = std::conditional_t<FX_RP16 != 0, uint16_t, uint32_t>;
-  using CiT = std::conditional_t<CD, int16_t, uint32_t>;
+  using CiT = std::conditional_t<CF == CF_D16, int16_t, uint32_t>;
+  static constexpr uint32_t kCi = CF == CF_PAT ? 1 : kBlockNnz;
+  static constexpr uint32_t kDict = CF == CF_PAT ? kPatDict : 1;
   RpT rp[kBlockRows + 1];
-  CiT ci[kBlockNnz];
+  CiT ci[kCi];
+  int32_t dict[kDict];
   MT va[kBlockNnz];
   __device__ __forceinline__ uint32_t col(uint32_t k, uint32_t r0) const {
-    if constexpr (CD) return (uint32_t)((int32_t)r0 + (int32_t)ci[k]);
+    if constexpr (CF == CF_D16) return (uint32_t)((int32_t)r0 + (int32_t)ci[k]);
     else return ci[k];
   }
 };
@@ -123,23 +139,24 @@ __device__ __forceinline__ T stream_load(const T* p) {
 // nz).  The entry loads are unconditional (indices past the block's end are
 // clamped to its last entry) and branch-free, and the row-pointer loads come
 // after them, so all of a thread's loads are in flight before the first
-// wait.  CD: 16-bit column deltas to r0; VA: the matrix has values.
-template <bool CD, bool VA, class MT, class VT>
-__device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CD>& lds,
+// wait.  CF: the column format (CF_U32 or CF_D16); VA: the matrix has
+// values.
+template <int CF, bool VA, class MT, class VT>
+__device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CF>& lds,
                                             uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
                                             int tid) {
   constexpr int kPer = kBlockNnz / kThreads;
   // last entry index a load may touch: the block's last entry, or for a
   // block of empty rows the matrix's last entry (nothing is stored then)
   const uint32_t klast = nz ? k0 + nz - 1 : in.nnz - 1;
-  typename SpmvLds<MT, CD>::CiT cix[kPer];
+  typename SpmvLds<MT, CF>::CiT cix[kPer];
   MT vv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
-    if constexpr (CD)
+    if constexpr (CF == CF_D16)
       cix[j] = stream_load(&in.cd[k]);
-    else
+    else if constexpr (CF == CF_U32)
       cix[j] = stream_load(&in.ci[k]);
     if constexpr (VA) vv[j] = stream_load(&in.va[k]);
   }
@@ -183,6 +200,135 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
 #endif
 }
 
+// A row-pattern block (CF_PAT: rows of at most kPatMaxLen entries, one per
+// thread, kBlockRows == kThreads).  The loads go out in the order: the
+// pattern dictionary (small, L2-resident), the row's pattern id and bounds,
+// then the block's values (f32), or the values first (f64).  Each row is
+// summed left to right as in the thread-per-row loop of k_spmv, with
+// column r + dict[pattern * plen + j].
+// f32 (early gathers): after a first barrier (the dictionary in LDS) the
+// row's first 8 x gathers are issued while the values are still in flight;
+// the dictionary stores and these gathers are branch-free (threads without
+// a row gather x[0]; every thread stores its own dictionary slots, those
+// past the dictionary's end with values never read), so that the waits for
+// the values never wait for the gathers.  f64: values first and the gathers after the one barrier,
+// which measured faster for the HBM-bound f64 SpMV (32.2 against 33.5 us
+// on the 128^3 stencil) though slower inside CG (20.4 against 20.3 ms);
+// the f32 order won every f32 measurement (IR 11.3 against 11.6 ms).
+template <bool VA, int MODE, int NRED, class MT, class VT, class XF, class BF, class Epi>
+__device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CF_PAT>& lds,
+                                          uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
+                                          int tid, const XF& xf, const BF& bf, const Epi& epi,
+                                          double (&acc)[NRED > 0 ? NRED : 1]) {
+  constexpr int kPer = kBlockNnz / kThreads;
+  constexpr int kDv = kPatDict / kThreads;
+  constexpr bool kEarly = sizeof(MT) == 4;
+  const uint32_t klast = nz ? k0 + nz - 1 : in.nnz - 1;
+  const bool has_row = (uint32_t)tid < nr;
+  const uint32_t row = has_row ? (uint32_t)tid : nr - 1;
+  int32_t dv[kDv];
+  MT vv[kPer];
+  uint32_t pidv = 0, s = 0, e = 0;
+  // f32: dictionary, pattern id, the row's bounds, then values
+  // f64: values, the row's start (to LDS), pattern id, dictionary (only
+  //      the slots it has)
+  if constexpr (kEarly) {
+#pragma unroll
+    for (int j = 0; j < kDv; ++j) {
+      const uint32_t i = tid + j * kThreads;
+      dv[j] = in.pdict[i < in.dlen ? i : in.dlen - 1];
+    }
+    pidv = in.pid[r0 + row];
+#if FX_RP16
+    s = in.rpb[r0 + row];
+    const uint32_t e_next = in.rpb[r0 + (row + 1 < nr ? row + 1 : row)];
+    e = row + 1 < nr ? e_next : nz;
+#else
+    s = in.rp[r0 + row] - k0;
+    e = in.rp[r0 + row + 1] - k0;
+#endif
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
+    vv[j] = VA ? stream_load(&in.va[k]) : (MT)1;
+  }
+  uint32_t rstart = 0;
+  if constexpr (!kEarly) {
+#if FX_RP16
+    rstart = in.rpb[r0 + row];
+#else
+    rstart = in.rp[r0 + row] - k0;
+#endif
+    pidv = in.pid[r0 + row];
+#pragma unroll
+    for (int j = 0; j < kDv; ++j) {
+      const uint32_t i = tid + j * kThreads;
+      if (i < in.dlen) dv[j] = in.pdict[i];
+    }
+  }
+  const uint32_t ri = r0 + row;
+  double xv[8];
+  if constexpr (kEarly) {
+#pragma unroll
+    for (int j = 0; j < kDv; ++j) lds.dict[tid + j * kThreads] = dv[j];  // slots >= dlen: unread
+    __syncthreads();
+    const bool live = has_row && e > s;
+    const int32_t* pd = lds.dict + (live ? pidv * in.plen : 0u);
+    const uint32_t lm1 = live ? e - s - 1 : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t q = min((uint32_t)j, lm1);
+      const uint32_t c = (uint32_t)((int32_t)ri + pd[q]);
+      xv[j] = (double)xf(live ? c : 0u);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kDv; ++j) {
+      const uint32_t i = tid + j * kThreads;
+      if (i < in.dlen) lds.dict[i] = dv[j];
+    }
+    if (has_row) lds.rp[row] = rstart;
+    if (tid == 0) lds.rp[nr] = nz;
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t k = tid + j * kThreads;
+    if (k < nz) lds.va[k] = vv[j];
+  }
+  __syncthreads();
+  if constexpr (!kEarly) {
+    s = lds.rp[row];
+    e = lds.rp[row + 1];
+  }
+  const int32_t* pd = lds.dict + pidv * in.plen;
+  if (!has_row) return;
+  double t = (MODE == ROW_AX) ? 0.0 : (double)bf(ri);
+  for (uint32_t base = s; base < e; base += 8) {
+    double av[8];
+    if (!kEarly || base != s) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t q = min(base + j, e - 1) - s;
+        xv[j] = (double)xf((uint32_t)((int32_t)ri + pd[q]));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av[j] = (double)lds.va[min(base + j, e - 1)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t k = base + j;
+      if (k < e) {
+        if (MODE == ROW_AX)
+          t = (k == s) ? av[j] * xv[j] : fma(av[j], xv[j], t);
+        else
+          t = fma(-av[j], xv[j], t);
+      }
+    }
+  }
+  epi.row(ri, t, acc);
+}
+
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
 // (a plain load, or a value formed on the fly); B gives b_i for
 // ROW_B_MINUS_AX.  Epi must provide:
@@ -190,7 +336,7 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
 //   void row(uint32_t r, double v, double (&acc)[NRED])
 //   void fin(double (&sum)[NRED])          run by k_finish after the kernel
 template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi,
-          bool CD>
+          int CF>
 __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF bf, Epi epi0,
                                                    double* partials) {
   if (epi0.skip()) return;
@@ -198,7 +344,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
   Epi epi = epi0;
   maybe_init(xf);
   maybe_init(epi);
-  __shared__ SpmvLds<MT, CD> lds;
+  __shared__ SpmvLds<MT, CF> lds;
   double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
   for (int k = 0; k < (NRED > 0 ? NRED : 1); ++k) acc[k] = 0.0;
@@ -219,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         for (int j = 0; j < 4; ++j) {
           const uint32_t k = min(kb + j * kThreads, k1 - 1);
           a[j] = in.va ? (double)in.va[k] : 1.0;
-          if constexpr (CD)
+          if constexpr (CF == CF_D16)
             c[j] = (uint32_t)((int32_t)r0 + (int32_t)in.cd[k]);
           else
             c[j] = in.ci[k];
@@ -236,9 +382,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         const double v = (MODE == ROW_AX) ? t[0] : (double)bf(r0) - t[0];
         epi.row(r0, v, acc);
       }
+    } else if constexpr (CF == CF_PAT) {
+      in.va ? pat_block<true, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc)
+            : pat_block<false, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc);
     } else {
-      in.va ? stage_block<CD, true>(in, lds, r0, k0, nr, nz, tid)
-            : stage_block<CD, false>(in, lds, r0, k0, nr, nz, tid);
+      in.va ? stage_block<CF, true>(in, lds, r0, k0, nr, nz, tid)
+            : stage_block<CF, false>(in, lds, r0, k0, nr, nz, tid);
       __syncthreads();
       const int tpr = threads_per_row(nr, nz);
       if (tpr == 1) {
@@ -387,18 +536,17 @@ struct EpiStore {
 
 // With events e0 / e1 the kernel is launched with them attached to its
 // dispatch (hipExtLaunchKernelGGL), so that they time the kernel alone.
-// The kernel is instantiated for 16-bit column deltas (CD) and for u32
-// columns; A->cd picks one.
+// The kernel is instantiated for each column format; A->pid / A->cd pick one.
 template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi,
-          bool CD>
+          int CF>
 inline void launch_spmv_k(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
                           hipEvent_t e0, hipEvent_t e1, unsigned g) {
   if (e0)
-    hipExtLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CD>), dim3(g),
+    hipExtLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF>), dim3(g),
                           dim3(kThreads), 0, st, e0, e1, 0, spmv_in<MT, VT>(A), xf, bf, epi,
                           partials);
   else
-    hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CD>), dim3(g),
+    hipLaunchKernelGGL((k_spmv<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF>), dim3(g),
                        dim3(kThreads), 0, st, spmv_in<MT, VT>(A), xf, bf, epi, partials);
 }
 
@@ -406,12 +554,15 @@ template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, cl
 inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finisher = true) {
   const unsigned g = spmv_grid(A->nblk, A->ctx->sms);
-  if (A->cd)
-    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, true>(A, st, xf, bf, epi, partials,
-                                                                  e0, e1, g);
+  if (A->pid)
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_PAT>(A, st, xf, bf, epi, partials,
+                                                                    e0, e1, g);
+  else if (A->cd)
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_D16>(A, st, xf, bf, epi, partials,
+                                                                    e0, e1, g);
   else
-    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, false>(A, st, xf, bf, epi, partials,
-                                                                   e0, e1, g);
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_U32>(A, st, xf, bf, epi, partials,
+                                                                    e0, e1, g);
   FX_HIP(hipGetLastError());
   if constexpr (NRED > 0) {
     if (finisher) return launch_finish<NRED, MAXMASK>(st, partials, g, epi);

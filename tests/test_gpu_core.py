@@ -138,10 +138,12 @@ def test_column_delta_format(ctx, problems, monkeypatch, cid, scale, nbytes):
     bits; the power-law matrix's random columns keep u32 once n exceeds the
     16-bit reach (at n = 16384 every column is within 32K of every row)."""
     pr = problems(cid, scale)
+    monkeypatch.setenv("FASPXX_FX_NOPAT", "1")  # row patterns off: deltas where they fit
     A = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.setenv("FASPXX_FX_COL32", "1")
     A32 = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.delenv("FASPXX_FX_COL32")
+    monkeypatch.delenv("FASPXX_FX_NOPAT")
     assert A.index_bytes == nbytes and A32.index_bytes == 4
     x = dev(ctx, np.random.default_rng(7).uniform(-1, 1, pr.A.nrow))
     w, w32 = fx.Vec(ctx, pr.A.nrow), fx.Vec(ctx, pr.A.nrow)
@@ -150,10 +152,11 @@ def test_column_delta_format(ctx, problems, monkeypatch, cid, scale, nbytes):
     assert np.array_equal(w.numpy(), w32.numpy())
 
 
-def test_column_delta_fallback(ctx, oracle):
+def test_column_delta_fallback(ctx, oracle, monkeypatch):
     """One column 39,999 rows away from its block start: the whole matrix
     keeps u32 columns, and the SpMV stays bit-exact."""
     from faspxx_amd import problems as P
+    monkeypatch.setenv("FASPXX_FX_NOPAT", "1")
     B = P.poisson2d(200)
     n = B.nrow
     rp = B.row_ptr.astype(np.int64)
@@ -167,6 +170,104 @@ def test_column_delta_fallback(ctx, oracle):
     x = np.random.default_rng(8).uniform(-1, 1, n)
     w = fx.Vec(ctx, n)
     A.Apply(dev(ctx, x), w)
+    assert np.array_equal(w.numpy(), oracle.spmv(Matrix(n, n, rp2, ci, va, dp), x))
+
+
+def _formats(ctx, A64, monkeypatch):
+    """The same matrix as row patterns (default), 16-bit deltas and u32."""
+    A = fx.Csr.from_problem(ctx, A64)
+    monkeypatch.setenv("FASPXX_FX_NOPAT", "1")
+    Ad = fx.Csr.from_problem(ctx, A64)
+    monkeypatch.setenv("FASPXX_FX_COL32", "1")
+    Au = fx.Csr.from_problem(ctx, A64)
+    monkeypatch.delenv("FASPXX_FX_COL32")
+    monkeypatch.delenv("FASPXX_FX_NOPAT")
+    return A, Ad, Au
+
+
+@pytest.mark.parametrize("cid,scale", [(1, None), (2, 32), (3, 256), (5, 64), (2, None)])
+def test_row_pattern_format(ctx, problems, oracle, monkeypatch, cid, scale):
+    """Stencil matrices repeat a few column-offset patterns (27 for the 3D
+    7-point stencil): they are stored as one pattern id per row, and
+    Apply (f64 and f32) and Residual give the same bits as with 16-bit
+    deltas and u32 columns, and as the oracle's SpMV."""
+    pr = problems(cid, scale)
+    A, Ad, Au = _formats(ctx, pr.A, monkeypatch)
+    assert (A.index_bytes, Ad.index_bytes, Au.index_bytes) == (0, 2, 4)
+    n = pr.A.nrow
+    xh = np.random.default_rng(11).uniform(-1, 1, n)
+    x = dev(ctx, xh)
+    outs = []
+    for M in (A, Ad, Au):
+        w = fx.Vec(ctx, n)
+        M.Apply(x, w)
+        outs.append(w.numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    if n <= 1 << 17:
+        assert np.array_equal(outs[0], oracle.spmv(Matrix.of(pr.A), xh))
+    b = dev(ctx, pr.b)
+    res = []
+    for M in (A, Au):
+        r = fx.Vec(ctx, n)
+        M.Residual(b, x, r)
+        res.append(r.numpy())
+    assert np.array_equal(res[0], res[1])
+    f32 = []
+    x32 = dev(ctx, xh.astype(np.float32), fx.F32)
+    for M in (A, Au):
+        M.make_f32()
+        w = fx.Vec(ctx, n, fx.F32)
+        M.Apply(x32, w)
+        f32.append(w.numpy())
+    assert np.array_equal(f32[0], f32[1])
+
+
+def _two_entry_rows(n, K):
+    """Row i < n - K - 1: columns i and i + 1 + (i mod K); later rows: i
+    alone.  K + 1 distinct row patterns."""
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        cols = [i] if i >= n - K - 1 else [i, i + 1 + (i % K)]
+        ci += cols
+        va += [4.0] + [-1.0] * (len(cols) - 1)
+        rp.append(len(ci))
+    return (np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va))
+
+
+@pytest.mark.parametrize("K,fmt", [(254, 0), (255, 2)])
+def test_row_pattern_limits(ctx, oracle, K, fmt):
+    """255 distinct row patterns fit the 1-byte ids, 256 do not (the
+    matrix keeps 16-bit deltas); either way the SpMV is bit-exact."""
+    n = 4000
+    rp, ci, va = _two_entry_rows(n, K)
+    A = fx.Csr(ctx, n, n, rp, ci, va, None)
+    assert A.index_bytes == fmt
+    x = np.random.default_rng(12).uniform(-1, 1, n)
+    w = fx.Vec(ctx, n)
+    A.Apply(dev(ctx, x), w)
+    dp = oracle.form_diagptr(n, rp, ci)
+    assert np.array_equal(w.numpy(), oracle.spmv(Matrix(n, n, rp, ci, va, dp), x))
+
+
+def test_row_pattern_long_row(ctx, oracle):
+    """A row of 33 entries (longer than the pattern format takes) keeps the
+    matrix on 16-bit deltas; the SpMV stays bit-exact."""
+    from faspxx_amd import problems as P
+    B = P.poisson2d(64)
+    n = B.nrow
+    rp = B.row_ptr.astype(np.int64)
+    extra = [c for c in range(40, 73)]  # row 5 gets 33 entries: columns 40..72
+    ci = list(B.col_ind[: rp[5]]) + extra + list(B.col_ind[rp[6]:])
+    va = list(B.values[: rp[5]]) + [-0.25] * len(extra) + list(B.values[rp[6]:])
+    d = len(extra) - (rp[6] - rp[5])
+    rp2 = np.concatenate([rp[:6], rp[6:] + d]).astype(np.uint32)
+    ci, va = np.array(ci, np.uint32), np.array(va)
+    A = fx.Csr(ctx, n, n, rp2, ci, va, None)
+    assert A.index_bytes == 2
+    x = np.random.default_rng(13).uniform(-1, 1, n)
+    w = fx.Vec(ctx, n)
+    A.Apply(dev(ctx, x), w)
+    dp = oracle.form_diagptr(n, rp2, ci)
     assert np.array_equal(w.numpy(), oracle.spmv(Matrix(n, n, rp2, ci, va, dp), x))
 
 
