@@ -180,11 +180,14 @@ int fx_csr_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, uint32_t nnz,
                   const uint32_t* diagPtr, fx_csr* out);
 int fx_csr_destroy(fx_csr A);
 int fx_csr_info(fx_csr A, uint32_t* nrow, uint32_t* mcol, uint32_t* nnz);
-/* Bytes per column index that the SpMV kernels read: 2 when every row
- * block's columns lie within 32K of the block's first row (banded and
- * stencil matrices are stored with 16-bit deltas), else 4; environment
- * FASPXX_FX_COL32=1 at creation keeps 4.  No reference counterpart (the
- * reference stores u32 columns only, src/MAT.hxx:43-47). */
+/* Bytes per column index that the SpMV kernels read: 0 for row patterns
+ * (a matrix created from host arrays whose rows repeat at most 255
+ * column-offset patterns of at most 32 entries, with at most 1024 offsets in
+ * all: one pattern byte per row instead), 2 when every row block's columns
+ * lie within 32K of the block's first row (16-bit deltas), else 4.
+ * Environment at creation: FASPXX_FX_NOPAT=1 disables row patterns,
+ * FASPXX_FX_COL32=1 keeps 4.  All forms give the same bits.  No reference
+ * counterpart (the reference stores u32 columns only, src/MAT.hxx:43-47). */
 int fx_csr_index_bytes(fx_csr A, int* bytes);
 /* Copy the device arrays back (any pointer may be NULL). */
 int fx_csr_download(fx_csr A, uint32_t* rowPtr, uint32_t* colInd, double* values,
