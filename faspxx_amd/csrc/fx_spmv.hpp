@@ -6,7 +6,9 @@
 //   * the block's row pointers are staged in LDS;
 //   * its column indices and values are read once, fully coalesced, into LDS;
 //     the columns come as 2-byte deltas to the block's first row when the
-//     matrix has them (banded / stencil matrices), else as u32;
+//     matrix has them (banded / stencil matrices), else as u32; a matrix
+//     stored as row patterns (stencils) stages only its values and the small
+//     pattern dictionary, and forms each column as row + offset (pat_block);
 //   * rows are then reduced with x gathered straight from memory.  While the
 //     block's rows average at most 32 entries each row is summed by one
 //     thread, left to right, in the reference's order (w = v0*x0, then
