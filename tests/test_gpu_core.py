@@ -271,6 +271,45 @@ def test_row_pattern_long_row(ctx, oracle):
     assert np.array_equal(w.numpy(), oracle.spmv(Matrix(n, n, rp2, ci, va, dp), x))
 
 
+def test_row_pattern_rectangular_empty_rows(ctx, oracle, monkeypatch):
+    """Row patterns on a rectangular matrix (3000 x 3100) whose every 7th row
+    is empty, with values and structure-only: the same bits as u32 columns
+    and as the oracle; empty rows give 0; a structure-only row is the sum of
+    x over its columns, left to right."""
+    n, m = 3000, 3100
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        cols = [] if i % 7 == 3 else [c for c in (i - 2, i, i + 50, i + 100) if c >= 0]
+        ci += cols
+        va += [float(1 + i % 5)] * len(cols[:1]) + [-0.5] * max(len(cols) - 1, 0)
+        rp.append(len(ci))
+    rp, ci, va = np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va)
+    x = np.random.default_rng(14).uniform(-1, 1, m)
+    dp = oracle.form_diagptr(n, rp, ci)
+    for vals in (va, None):
+        A = fx.Csr(ctx, n, m, rp, ci, vals, None)
+        monkeypatch.setenv("FASPXX_FX_COL32", "1")
+        Au = fx.Csr(ctx, n, m, rp, ci, vals, None)
+        monkeypatch.delenv("FASPXX_FX_COL32")
+        assert A.index_bytes == 0 and Au.index_bytes == 4
+        w, wu = fx.Vec(ctx, n), fx.Vec(ctx, n)
+        A.Apply(dev(ctx, x), w)
+        Au.Apply(dev(ctx, x), wu)
+        got = w.numpy()
+        assert np.array_equal(got, wu.numpy())
+        assert np.all(got[3::7] == 0.0)
+        if vals is not None:
+            assert np.array_equal(got, oracle.spmv(Matrix(n, m, rp, ci, va, dp), x))
+        else:
+            ref = np.zeros(n)
+            for i in range(n):
+                t = 0.0
+                for k in range(rp[i], rp[i + 1]):
+                    t += x[ci[k]]
+                ref[i] = t
+            assert np.array_equal(got, ref)
+
+
 def test_spmv_f32(ctx, problems):
     pr = problems(2, 32)
     A = fx.Csr.from_problem(ctx, pr.A)
