@@ -14,7 +14,9 @@ namespace fx {
 namespace {
 
 // Row-block table (see fx_csr_s).
-std::vector<uint32_t> build_row_blocks(uint32_t nrow, const uint32_t* rp, uint32_t* max_row) {
+std::vector<uint32_t> build_row_blocks(uint32_t nrow, const uint32_t* rp, uint32_t* max_row,
+                                       uint32_t rows_cap = kBlockRows,
+                                       uint32_t nnz_cap = kBlockNnz) {
   std::vector<uint32_t> blk;
   blk.reserve(nrow / 128 + 2);
   blk.push_back(0);
@@ -23,14 +25,14 @@ std::vector<uint32_t> build_row_blocks(uint32_t nrow, const uint32_t* rp, uint32
     const uint32_t start = r;
     const uint32_t len = rp[r + 1] - rp[r];
     if (len > mx) mx = len;
-    if (len > kBlockNnz) {
+    if (len > nnz_cap) {
       ++r;
     } else {
       uint32_t nz = 0;
-      while (r < nrow && r - start < kBlockRows) {
+      while (r < nrow && r - start < rows_cap) {
         const uint32_t l = rp[r + 1] - rp[r];
         if (l > mx) mx = l;
-        if (nz + l > kBlockNnz) break;
+        if (nz + l > nnz_cap) break;
         nz += l;
         ++r;
       }
@@ -274,8 +276,23 @@ bool patterns_disabled() {
 }
 
 int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
+  // Row patterns for a matrix given on the host (FASPXX_FX_NOPAT=1: none;
+  // FASPXX_FX_COL32=1 keeps u32 columns); they get their own row blocks.
+  const char* e = std::getenv("FASPXX_FX_COL32");
+  const bool col32 = e && e[0] == '1';
+  std::vector<uint8_t> pid;
+  std::vector<int32_t> dict;
+  uint32_t plen = 0, npat = 0;
+  bool pat = false;
+  if (host_ci && A->nnz && !col32 && !patterns_disabled()) {
+    uint32_t longest = 0;
+    for (uint32_t r = 0; r < A->nrow; ++r) longest = std::max(longest, host_rp[r + 1] - host_rp[r]);
+    pat = build_patterns(A->nrow, host_rp, host_ci, longest, pid, dict, plen, npat);
+  }
   uint32_t mx = 0;
-  const std::vector<uint32_t> rows = build_row_blocks(A->nrow, host_rp, &mx);
+  const std::vector<uint32_t> rows =
+      pat ? build_row_blocks(A->nrow, host_rp, &mx, kPatRows, kPatNnz)
+          : build_row_blocks(A->nrow, host_rp, &mx);
   A->nblk = (uint32_t)rows.size() - 1;
   A->max_row = mx;
   std::vector<uint32_t> blk(2 * rows.size());
@@ -294,26 +311,18 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
   FX_HIP(hipMalloc(&A->rpb, sizeof(uint16_t) * rpb.size()));
   FX_HIP(hipMemcpyAsync(A->rpb, rpb.data(), sizeof(uint16_t) * rpb.size(), hipMemcpyHostToDevice,
                         st));
-  // 2-byte column deltas when every block admits them (FASPXX_FX_COL32=1 keeps u32 columns);
-  // before them, row patterns for a matrix given on the host (FASPXX_FX_NOPAT=1: none)
-  const char* e = std::getenv("FASPXX_FX_COL32");
-  const bool col32 = e && e[0] == '1';
-  if (host_ci && A->nnz && A->nblk && !col32 && !patterns_disabled() && kBlockRows == kThreads) {
-    std::vector<uint8_t> pid;
-    std::vector<int32_t> dict;
-    uint32_t plen = 0, npat = 0;
-    if (build_patterns(A->nrow, host_rp, host_ci, mx, pid, dict, plen, npat)) {
-      FX_HIP(hipMalloc(&A->pid, pid.size()));
-      FX_HIP(hipMalloc(&A->pdict, sizeof(int32_t) * dict.size()));
-      FX_HIP(hipMemcpyAsync(A->pid, pid.data(), pid.size(), hipMemcpyHostToDevice, st));
-      FX_HIP(hipMemcpyAsync(A->pdict, dict.data(), sizeof(int32_t) * dict.size(),
-                            hipMemcpyHostToDevice, st));
-      A->plen = plen;
-      A->npat = npat;
-      FX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
-      return 0;
-    }
+  if (pat) {
+    FX_HIP(hipMalloc(&A->pid, pid.size()));
+    FX_HIP(hipMalloc(&A->pdict, sizeof(int32_t) * dict.size()));
+    FX_HIP(hipMemcpyAsync(A->pid, pid.data(), pid.size(), hipMemcpyHostToDevice, st));
+    FX_HIP(hipMemcpyAsync(A->pdict, dict.data(), sizeof(int32_t) * dict.size(),
+                          hipMemcpyHostToDevice, st));
+    A->plen = plen;
+    A->npat = npat;
+    FX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+    return 0;
   }
+  // 2-byte column deltas when every block admits them
   if (A->nnz && A->nblk && !col32) {
     uint32_t* bad = nullptr;
     FX_HIP(hipMalloc(&A->cd, sizeof(int16_t) * A->nnz));
@@ -358,6 +367,31 @@ void csr_free(fx_csr A) {
 }  // namespace
 
 int csr_storage(fx_csr A) { return csr_alloc(A); }
+
+int csr_pattern_blocks512(fx_csr A, uint32_t** blk, uint16_t** rpb, uint32_t* nblk) {
+  hipStream_t st = A->ctx->stream;
+  std::vector<uint32_t> hrp((size_t)A->nrow + 1);
+  FX_HIP(hipMemcpyAsync(hrp.data(), A->rp, sizeof(uint32_t) * hrp.size(), hipMemcpyDeviceToHost,
+                        st));
+  FX_HIP(hipStreamSynchronize(st));
+  uint32_t mx = 0;
+  const std::vector<uint32_t> rows = build_row_blocks(A->nrow, hrp.data(), &mx, kPat2Rows, kPat2Nnz);
+  std::vector<uint32_t> hb(2 * rows.size());
+  for (size_t b = 0; b < rows.size(); ++b) {
+    hb[2 * b] = rows[b];
+    hb[2 * b + 1] = hrp[rows[b]];
+  }
+  std::vector<uint16_t> hr(A->nrow ? A->nrow : 1, 0);
+  for (size_t b = 0; b + 1 < rows.size(); ++b)
+    for (uint32_t r = rows[b]; r < rows[b + 1]; ++r) hr[r] = (uint16_t)(hrp[r] - hrp[rows[b]]);
+  FX_HIP(hipMalloc(blk, sizeof(uint32_t) * hb.size()));
+  FX_HIP(hipMalloc(rpb, sizeof(uint16_t) * hr.size()));
+  FX_HIP(hipMemcpyAsync(*blk, hb.data(), sizeof(uint32_t) * hb.size(), hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(*rpb, hr.data(), sizeof(uint16_t) * hr.size(), hipMemcpyHostToDevice, st));
+  FX_HIP(hipStreamSynchronize(st));
+  *nblk = (uint32_t)rows.size() - 1;
+  return 0;
+}
 
 void csr_release(fx_csr A) { csr_free(A); }
 

@@ -120,6 +120,7 @@ struct fx_csr_s {
   uint8_t* pid = nullptr;
   int32_t* pdict = nullptr;
   uint32_t plen = 0, npat = 0;
+  uint32_t pat_rows = 256;  // rows per row block of a row-pattern matrix (256 or 512)
   // set on a transpose: entry p of this matrix is entry src[p] of its source
   uint32_t* src = nullptr;
 };
@@ -152,7 +153,14 @@ static_assert(kBlockRows % kThreads == 0, "row blocks hold whole passes of the w
 constexpr uint32_t kBlockNnz = FX_BLOCK_NNZ;
 constexpr uint32_t kPatMaxLen = 32;  // longest row the row-pattern format takes
 constexpr uint32_t kPatMax = 255;    // most patterns it takes (1-byte ids)
-constexpr uint32_t kPatDict = 1024;  // largest dictionary (patterns x plen offsets) it takes
+constexpr uint32_t kPatDict = 512;   // largest dictionary (patterns x plen offsets) it takes
+// Row blocks of a row-pattern matrix: 256 rows of up to 7 entries on
+// average, as the general blocks; or, for the refinement's f32 copy of the
+// matrix (pat_rows == 512), 512 rows (two per thread).
+constexpr uint32_t kPatRows = 256;
+constexpr uint32_t kPatNnz = 7 * kPatRows;
+constexpr uint32_t kPat2Rows = 512;
+constexpr uint32_t kPat2Nnz = 7 * kPat2Rows;
 
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
@@ -421,6 +429,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finish(const double* __restrict
 // filled, form diagPtr (first match, as FormDiagPtr) and the row-block
 // tables.
 int csr_storage(fx_csr A);
+// Row blocks of kPat2Rows rows for a row-pattern matrix (its row pointers
+// are read back from the device): the (first row, first entry) table, the
+// 2-byte block-relative row pointers and the block count, in new device
+// allocations owned by the caller.
+int csr_pattern_blocks512(fx_csr A, uint32_t** blk, uint16_t** rpb, uint32_t* nblk);
 void csr_release(fx_csr A);
 int scan_u32(hipStream_t st, const uint32_t* cnt, uint32_t m, uint32_t* out);
 int csr_finish_device(fx_csr A);

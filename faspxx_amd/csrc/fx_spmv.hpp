@@ -50,8 +50,10 @@ struct SpmvIn {
 };
 
 // Column formats of the staged entries: u32 columns, 16-bit deltas to the
-// block's first row, or row patterns (no per-entry column data).
-enum { CF_U32 = 0, CF_D16 = 1, CF_PAT = 2 };
+// block's first row, or row patterns (no per-entry column data) in blocks
+// of 256 rows (CF_PAT) or 512 rows (CF_PAT2).
+enum { CF_U32 = 0, CF_D16 = 1, CF_PAT = 2, CF_PAT2 = 3 };
+constexpr bool is_pat(int cf) { return cf == CF_PAT || cf == CF_PAT2; }
 
 template <class MT, class VT>
 inline SpmvIn<MT, VT> spmv_in(fx_csr A);
@@ -120,12 +122,14 @@ template <class MT, int CF>
 struct SpmvLds {
   using RpT = std::conditional_t<FX_RP16 != 0, uint16_t, uint32_t>;
   using CiT = std::conditional_t<CF == CF_D16, int16_t, uint32_t>;
-  static constexpr uint32_t kCi = CF == CF_PAT ? 1 : kBlockNnz;
-  static constexpr uint32_t kDict = CF == CF_PAT ? kPatDict : 1;
-  RpT rp[kBlockRows + 1];
+  static constexpr uint32_t kRows = CF == CF_PAT2 ? kPat2Rows : CF == CF_PAT ? kPatRows : kBlockRows;
+  static constexpr uint32_t kNnz = CF == CF_PAT2 ? kPat2Nnz : CF == CF_PAT ? kPatNnz : kBlockNnz;
+  static constexpr uint32_t kCi = is_pat(CF) ? 1 : kBlockNnz;
+  static constexpr uint32_t kDict = is_pat(CF) ? kPatDict : 1;
+  RpT rp[kRows + 1];
   CiT ci[kCi];
   int32_t dict[kDict];
-  MT va[kBlockNnz];
+  MT va[kNnz];
   __device__ __forceinline__ uint32_t col(uint32_t k, uint32_t r0) const {
     if constexpr (CF == CF_D16) return (uint32_t)((int32_t)r0 + (int32_t)ci[k]);
     else return ci[k];
@@ -208,37 +212,46 @@ __device__ __forceinline__ void stage_block(const SpmvIn<MT, VT>& in, SpmvLds<MT
 #endif
 }
 
-// A row-pattern block (CF_PAT: rows of at most kPatMaxLen entries, one per
-// thread, kBlockRows == kThreads).  The loads go out in the order: the
+// A row-pattern block (CF_PAT / CF_PAT2: at most 256 / 512 rows of at most
+// kPatMaxLen entries, one / two rows per thread).  The loads go out in the
+// order: the
 // pattern dictionary (small, L2-resident), the row's pattern id and bounds,
 // then the block's values (f32), or the values first (f64).  Each row is
 // summed left to right as in the thread-per-row loop of k_spmv, with
 // column r + dict[pattern * plen + j].
-// f32 (early gathers): after a first barrier (the dictionary in LDS) the
+// f32 (early gathers): after a first barrier (the dictionary in LDS) each
 // row's first 8 x gathers are issued while the values are still in flight;
 // the dictionary stores and these gathers are branch-free (threads without
 // a row gather x[0]; every thread stores its own dictionary slots, those
 // past the dictionary's end with values never read), so that the waits for
-// the values never wait for the gathers.  f64: values first and the gathers after the one barrier,
+// the values never wait for the gathers.  f64: values first and the
+// gathers after the one barrier,
 // which measured faster for the HBM-bound f64 SpMV (32.2 against 33.5 us
 // on the 128^3 stencil) though slower inside CG (20.4 against 20.3 ms);
 // the f32 order won every f32 measurement (IR 11.3 against 11.6 ms).
-template <bool VA, int MODE, int NRED, class MT, class VT, class XF, class BF, class Epi>
-__device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CF_PAT>& lds,
+template <int CF, bool VA, int MODE, int NRED, class MT, class VT, class XF, class BF, class Epi>
+__device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, CF>& lds,
                                           uint32_t r0, uint32_t k0, uint32_t nr, uint32_t nz,
                                           int tid, const XF& xf, const BF& bf, const Epi& epi,
                                           double (&acc)[NRED > 0 ? NRED : 1]) {
-  constexpr int kPer = kBlockNnz / kThreads;
+  constexpr int R = SpmvLds<MT, CF>::kRows / kThreads;  // rows per thread: tid + q * kThreads
+  constexpr int kPer = SpmvLds<MT, CF>::kNnz / kThreads;
   constexpr int kDv = kPatDict / kThreads;
   constexpr bool kEarly = sizeof(MT) == 4;
   const uint32_t klast = nz ? k0 + nz - 1 : in.nnz - 1;
-  const bool has_row = (uint32_t)tid < nr;
-  const uint32_t row = has_row ? (uint32_t)tid : nr - 1;
+  bool has[R];
+  uint32_t row[R], pidv[R], s[R], e[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const uint32_t rq = tid + q * kThreads;
+    has[q] = rq < nr;
+    row[q] = has[q] ? rq : nr - 1;
+    s[q] = e[q] = 0;
+  }
   int32_t dv[kDv];
   MT vv[kPer];
-  uint32_t pidv = 0, s = 0, e = 0;
-  // f32: dictionary, pattern id, the row's bounds, then values
-  // f64: values, the row's start (to LDS), pattern id, dictionary (only
+  // f32: dictionary, pattern ids, the rows' bounds, then values
+  // f64: values, the rows' starts (to LDS), pattern ids, dictionary (only
   //      the slots it has)
   if constexpr (kEarly) {
 #pragma unroll
@@ -246,49 +259,57 @@ __device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, 
       const uint32_t i = tid + j * kThreads;
       dv[j] = in.pdict[i < in.dlen ? i : in.dlen - 1];
     }
-    pidv = in.pid[r0 + row];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      pidv[q] = in.pid[r0 + row[q]];
 #if FX_RP16
-    s = in.rpb[r0 + row];
-    const uint32_t e_next = in.rpb[r0 + (row + 1 < nr ? row + 1 : row)];
-    e = row + 1 < nr ? e_next : nz;
+      s[q] = in.rpb[r0 + row[q]];
+      const uint32_t e_next = in.rpb[r0 + (row[q] + 1 < nr ? row[q] + 1 : row[q])];
+      e[q] = row[q] + 1 < nr ? e_next : nz;
 #else
-    s = in.rp[r0 + row] - k0;
-    e = in.rp[r0 + row + 1] - k0;
+      s[q] = in.rp[r0 + row[q]] - k0;
+      e[q] = in.rp[r0 + row[q] + 1] - k0;
 #endif
+    }
   }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t k = min(k0 + (uint32_t)(tid + j * kThreads), klast);
     vv[j] = VA ? stream_load(&in.va[k]) : (MT)1;
   }
-  uint32_t rstart = 0;
+  uint32_t rstart[R];
   if constexpr (!kEarly) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
 #if FX_RP16
-    rstart = in.rpb[r0 + row];
+      rstart[q] = in.rpb[r0 + row[q]];
 #else
-    rstart = in.rp[r0 + row] - k0;
+      rstart[q] = in.rp[r0 + row[q]] - k0;
 #endif
-    pidv = in.pid[r0 + row];
+      pidv[q] = in.pid[r0 + row[q]];
+    }
 #pragma unroll
     for (int j = 0; j < kDv; ++j) {
       const uint32_t i = tid + j * kThreads;
       if (i < in.dlen) dv[j] = in.pdict[i];
     }
   }
-  const uint32_t ri = r0 + row;
-  double xv[8];
+  double xv[R][8];
   if constexpr (kEarly) {
 #pragma unroll
     for (int j = 0; j < kDv; ++j) lds.dict[tid + j * kThreads] = dv[j];  // slots >= dlen: unread
     __syncthreads();
-    const bool live = has_row && e > s;
-    const int32_t* pd = lds.dict + (live ? pidv * in.plen : 0u);
-    const uint32_t lm1 = live ? e - s - 1 : 0u;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t q = min((uint32_t)j, lm1);
-      const uint32_t c = (uint32_t)((int32_t)ri + pd[q]);
-      xv[j] = (double)xf(live ? c : 0u);
+    for (int q = 0; q < R; ++q) {
+      const bool live = has[q] && e[q] > s[q];
+      const int32_t* pd = lds.dict + (live ? pidv[q] * in.plen : 0u);
+      const uint32_t lm1 = live ? e[q] - s[q] - 1 : 0u;
+      const uint32_t ri = r0 + row[q];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t c = (uint32_t)((int32_t)ri + pd[min((uint32_t)j, lm1)]);
+        xv[q][j] = (double)xf(live ? c : 0u);
+      }
     }
   } else {
 #pragma unroll
@@ -296,7 +317,9 @@ __device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, 
       const uint32_t i = tid + j * kThreads;
       if (i < in.dlen) lds.dict[i] = dv[j];
     }
-    if (has_row) lds.rp[row] = rstart;
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      if (has[q]) lds.rp[row[q]] = rstart[q];
     if (tid == 0) lds.rp[nr] = nz;
   }
 #pragma unroll
@@ -306,35 +329,54 @@ __device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, 
   }
   __syncthreads();
   if constexpr (!kEarly) {
-    s = lds.rp[row];
-    e = lds.rp[row + 1];
-  }
-  const int32_t* pd = lds.dict + pidv * in.plen;
-  if (!has_row) return;
-  double t = (MODE == ROW_AX) ? 0.0 : (double)bf(ri);
-  for (uint32_t base = s; base < e; base += 8) {
-    double av[8];
-    if (!kEarly || base != s) {
+    // every row's first chunk of gathers goes out before any sum
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      s[q] = lds.rp[row[q]];
+      e[q] = lds.rp[row[q] + 1];
+      const bool live = has[q] && e[q] > s[q];
+      const int32_t* pd = lds.dict + (live ? pidv[q] * in.plen : 0u);
+      const uint32_t lm1 = live ? e[q] - s[q] - 1 : 0u;
+      const uint32_t ri = r0 + row[q];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const uint32_t q = min(base + j, e - 1) - s;
-        xv[j] = (double)xf((uint32_t)((int32_t)ri + pd[q]));
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) av[j] = (double)lds.va[min(base + j, e - 1)];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t k = base + j;
-      if (k < e) {
-        if (MODE == ROW_AX)
-          t = (k == s) ? av[j] * xv[j] : fma(av[j], xv[j], t);
-        else
-          t = fma(-av[j], xv[j], t);
+        const uint32_t c = (uint32_t)((int32_t)ri + pd[min((uint32_t)j, lm1)]);
+        xv[q][j] = (double)xf(live ? c : 0u);
       }
     }
   }
-  epi.row(ri, t, acc);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (!has[q]) continue;
+    const uint32_t ri = r0 + row[q], sq = s[q], eq = e[q];
+    const int32_t* pd = lds.dict + pidv[q] * in.plen;
+    double t = (MODE == ROW_AX) ? 0.0 : (double)bf(ri);
+    for (uint32_t base = sq; base < eq; base += 8) {
+      double av[8], xw[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (base == sq) {
+          xw[j] = xv[q][j];
+        } else {
+          const uint32_t k = min(base + j, eq - 1) - sq;
+          xw[j] = (double)xf((uint32_t)((int32_t)ri + pd[k]));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) av[j] = (double)lds.va[min(base + j, eq - 1)];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t k = base + j;
+        if (k < eq) {
+          if (MODE == ROW_AX)
+            t = (k == sq) ? av[j] * xw[j] : fma(av[j], xw[j], t);
+          else
+            t = fma(-av[j], xw[j], t);
+        }
+      }
+    }
+    epi.row(ri, t, acc);
+  }
 }
 
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
@@ -362,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
     const uint32_t r0 = in.blk[2 * bid], k0 = in.blk[2 * bid + 1];
     const uint32_t r1 = in.blk[2 * bid + 2], k1 = in.blk[2 * bid + 3];
     const uint32_t nr = r1 - r0, nz = k1 - k0;
-    if (nz > kBlockNnz) {
+    if (!is_pat(CF) && nz > kBlockNnz) {  // (row-pattern rows are short)
       // a single long row: strided partial sums (4 strides' loads in
       // flight at a time, summed in stride order), then a workgroup reduction
       double s = 0.0;
@@ -390,9 +432,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         const double v = (MODE == ROW_AX) ? t[0] : (double)bf(r0) - t[0];
         epi.row(r0, v, acc);
       }
-    } else if constexpr (CF == CF_PAT) {
-      in.va ? pat_block<true, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc)
-            : pat_block<false, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc);
+    } else if constexpr (is_pat(CF)) {
+      in.va ? pat_block<CF, true, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc)
+            : pat_block<CF, false, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc);
     } else {
       in.va ? stage_block<CF, true>(in, lds, r0, k0, nr, nz, tid)
             : stage_block<CF, false>(in, lds, r0, k0, nr, nz, tid);
@@ -562,7 +604,10 @@ template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, cl
 inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finisher = true) {
   const unsigned g = spmv_grid(A->nblk, A->ctx->sms);
-  if (A->pid)
+  if (A->pid && sizeof(MT) == 4 && A->pat_rows == kPat2Rows)  // f32 copies only
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi,
+                  sizeof(MT) == 4 ? CF_PAT2 : CF_PAT>(A, st, xf, bf, epi, partials, e0, e1, g);
+  else if (A->pid)
     launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_PAT>(A, st, xf, bf, epi, partials,
                                                                     e0, e1, g);
   else if (A->cd)

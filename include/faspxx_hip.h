@@ -182,7 +182,7 @@ int fx_csr_destroy(fx_csr A);
 int fx_csr_info(fx_csr A, uint32_t* nrow, uint32_t* mcol, uint32_t* nnz);
 /* Bytes per column index that the SpMV kernels read: 0 for row patterns
  * (a matrix created from host arrays whose rows repeat at most 255
- * column-offset patterns of at most 32 entries, with at most 1024 offsets in
+ * column-offset patterns of at most 32 entries, with at most 512 offsets in
  * all: one pattern byte per row instead), 2 when every row block's columns
  * lie within 32K of the block's first row (16-bit deltas), else 4.
  * Environment at creation: FASPXX_FX_NOPAT=1 disables row patterns,
