@@ -156,6 +156,60 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ cnt,
   if (threadIdx.x == 0) out[m] = carry;
 }
 
+// Multi-workgroup exclusive scan (integers, so any order gives the same
+// sums): each workgroup scans a tile of kScanTile counts into out and writes
+// the tile's total; k_scan scans the totals; each tile then adds its offset.
+constexpr uint32_t kScanPer = 8, kScanTile = 1024 * kScanPer;
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ cnt, uint32_t m,
+                                                     uint32_t* __restrict__ out,
+                                                     uint32_t* __restrict__ sums) {
+  __shared__ uint32_t ws[16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t base = blockIdx.x * kScanTile + tid * kScanPer;
+  uint32_t v[kScanPer], run = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kScanPer; ++j) {
+    v[j] = base + j < m ? cnt[base + j] : 0u;
+    run += v[j];
+  }
+  uint32_t x = run;  // inclusive scan of the threads' sums within the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if ((int)lane >= off) x += y;
+  }
+  if (lane == 63) ws[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    uint32_t w = lane < 16 ? ws[lane] : 0u;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const uint32_t y = __shfl_up(w, off, 64);
+      if ((int)lane >= off) w += y;
+    }
+    if (lane < 16) ws[lane] = w;
+  }
+  __syncthreads();
+  uint32_t acc = (wid ? ws[wid - 1] : 0u) + x - run;
+#pragma unroll
+  for (uint32_t j = 0; j < kScanPer; ++j) {
+    if (base + j < m) out[base + j] = acc;
+    acc += v[j];
+  }
+  if (tid == 0) sums[blockIdx.x] = ws[15];
+}
+
+// out[i] += offs[tile of i] for i < m; out[m] = the total.
+__global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t m,
+                           const uint32_t* __restrict__ offs, uint32_t ntile) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t add = offs[b];
+  for (uint32_t i = b * kScanTile + threadIdx.x; i < m && i < (b + 1) * kScanTile; i += blockDim.x)
+    out[i] += add;
+  if (b == 0 && threadIdx.x == 0) out[m] = offs[ntile];
+}
+
 __global__ void k_row_ids(const uint32_t* __restrict__ rp, uint32_t nrow, uint32_t* __restrict__ rid) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= nrow) return;
@@ -396,8 +450,22 @@ int csr_pattern_blocks512(fx_csr A, uint32_t** blk, uint16_t** rpb, uint32_t* nb
 void csr_release(fx_csr A) { csr_free(A); }
 
 int scan_u32(hipStream_t st, const uint32_t* cnt, uint32_t m, uint32_t* out) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cnt, m, out);
-  FX_HIP(hipGetLastError());
+  if (m <= kScanTile) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cnt, m, out);
+    FX_HIP(hipGetLastError());
+    return 0;
+  }
+  // 2M counts: 3.7 ms in one workgroup before
+  const uint32_t nt = (m + kScanTile - 1) / kScanTile;
+  uint32_t* tmp = nullptr;  // tile totals [0, nt), their scan [nt, 2 nt + 1)
+  FX_HIP(hipMallocAsync((void**)&tmp, sizeof(uint32_t) * (2 * (size_t)nt + 1), st));
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(1024), 0, st, cnt, m, out, tmp);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const uint32_t*)tmp, nt, tmp + nt);
+  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(1024), 0, st, out, m, (const uint32_t*)(tmp + nt),
+                     nt);
+  const hipError_t e = hipGetLastError();
+  FX_HIP(hipFreeAsync(tmp, st));
+  FX_HIP(e);
   return 0;
 }
 
