@@ -210,14 +210,20 @@ __global__ void k_mult_rows(const uint32_t* __restrict__ lrp, const uint32_t* __
   cnt[i] = c;
 }
 
+// Compaction of the product rows into CSR: 32 lanes per row, so that a
+// wave's loads and stores are consecutive (one thread per row took 3.1 ms
+// for the 51.7M entries of the 128^3 stencil's square).
+constexpr uint32_t kCopyLanes = 32;
 __global__ void k_mult_copy(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tcol,
                             const double* __restrict__ tval, uint32_t nrow,
                             const uint32_t* __restrict__ orp, uint32_t* __restrict__ oci,
                             double* __restrict__ ova) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t i = g / kCopyLanes;
+  const uint32_t lane = (uint32_t)(g % kCopyLanes);
   if (i >= nrow) return;
   const uint32_t s = off[i], o = orp[i], len = orp[i + 1] - o;
-  for (uint32_t j = 0; j < len; ++j) {
+  for (uint32_t j = lane; j < len; j += kCopyLanes) {
     oci[o + j] = tcol[s + j];
     if (tval) ova[o + j] = tval[s + j];
   }
@@ -437,8 +443,10 @@ int fx_csr_mult(fx_ctx ctx, fx_csr L, fx_csr R, fx_csr* C) {
   FX_TRY(scan_u32(st, cnt, n, orp));
   fx_csr P = nullptr;
   FX_TRY(new_from_rowptr(ctx, n, R->mcol, values, orp, &P));
-  hipLaunchKernelGGL(k_mult_copy, dim3(row_blocks(n)), dim3(kThreads), 0, st, off, tcol, tval, n,
-                     P->rp, P->ci, values ? P->va : nullptr);
+  const uint64_t copy_threads = (uint64_t)n * kCopyLanes;
+  hipLaunchKernelGGL(k_mult_copy, dim3((unsigned)((copy_threads + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, off, tcol, tval, n, P->rp, P->ci,
+                     values ? P->va : nullptr);
   return finish(P, C);
 }
 
