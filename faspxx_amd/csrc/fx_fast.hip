@@ -257,9 +257,10 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
   block_partials_n<5, 4u, kWideThreads>(acc, p2);
 }
 
-// Packs per thread per pass of K3.
+// Packs per thread per pass of K3: 2 measured 20.1 against 20.3 ms for the
+// FP64 CG on the 128^3 stencil (8: 20.06 ms, but 256 VGPRs and spills).
 #ifndef FX_PPACKS
-#define FX_PPACKS 4
+#define FX_PPACKS 2
 #endif
 constexpr int kPPacks = FX_PPACKS;
 template <class VT, bool JAC>
