@@ -3,12 +3,15 @@
 One step = one solve to a relative residual of 1e-8 from x0 = 0 with the
 FP32-inner / FP64-refinement CG (Jacobi preconditioned: the inner CG runs
 on the diagonally scaled matrix S A S, S = diag(a_ii)^-1/2, which is
-Jacobi-PCG in exact arithmetic), inputs resident in HBM.  Reported beside it, in the same run: FP64 Jacobi-PCG time-to-solution,
-standalone SpMV GB/s (f64 and f32, Infinity Cache flushed between reps), 50
-weighted-Jacobi sweeps (omega = 2/3), and the compiled reference CPU path
-(oracle/_ref) solving the same system on one host core.
+Jacobi-PCG in exact arithmetic), inputs resident in HBM.  Reported beside
+it, in the same run: FP64 Jacobi-PCG time-to-solution and the refined
+solution's error against it, standalone SpMV GB/s (f64 and f32, Infinity
+Cache flushed between reps), 50 weighted-Jacobi sweeps (omega = 2/3), and
+the compiled reference CPU path (oracle/_ref) solving the same system on one
+host core (and its OpenMP build on the available cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --spmv-only [--spmv-workload powerlaw_1048576]   # counter runs
 """
 from __future__ import annotations
 
@@ -63,6 +66,35 @@ def spmv_flushed(fx, ctx, A, n, nnz, dt, flush, sink, warmup, reps):
             "index_bytes": A.index_bytes}
 
 
+def profiled_traffic(name: str, index_bytes: int, nnz: int):
+    """HBM bytes per launch of an SpMV, from a committed profiler counter
+    summary (FETCH_SIZE and WRITE_SIZE runs of `bench.py --spmv-only`, each
+    scaled by what the counter reads for a copy of known size); None when
+    the summary was taken on another matrix or column-index format."""
+    tpath = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        tr = json.load(fh)
+    if tr.get("index_bytes") == index_bytes and tr.get("nnz") == nnz:
+        return tr["traffic_bytes"]
+    return None
+
+
+def cpu_info() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cores_available": len(os.sched_getaffinity(0))}
+
+
 def relres(pr, x):
     return float(np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b))
 
@@ -85,16 +117,20 @@ def leg_convdiff(fx, P, ctx):
                                     (fx.GMRES, "gmres30", 2, 20000),
                                     (fx.FGMRES, "fgmres30_budget2000", 1, 2000)):
         s = fx.Solver(ctx, kind)
-        s.Setup(A, pc, fx.params(maxIter=maxit, relTol=1e-8, absTol=0.0, restart=30))
+        prm = fx.params(maxIter=maxit, relTol=1e-8, absTol=0.0, restart=30)
         ms = []
         for _ in range(reps):
+            # Setup before each solve, as Krylov() does: GMRES starts from the
+            # set-up restart length (the workspace and graphs are kept)
+            s.Setup(A, pc, prm)
             x.set(0.0)
             r = s.Solve(b, x)
             ms.append(r.solve_ms)
         out[name] = {"ms": round(min(ms), 3), "iters": r.numIter, "retcode": r.retcode,
                      "relres_fp64": relres(pr, x.numpy()),
                      "us_per_iter": round(min(ms) * 1e3 / max(r.numIter, 1), 1),
-                     "path": "per-operation" if kind == fx.FGMRES else "fused graph replay"}
+                     "path": ("fused graph replay" if s.last_path == "fused"
+                              else "per-operation")}
         log(f"convdiff {name}: {min(ms):.1f} ms, {r.numIter} iters, relres "
             f"{out[name]['relres_fp64']:.2e}")
         s.close()
@@ -123,7 +159,8 @@ def leg_powerlaw(fx, P, ctx, flush, sink, warmup, reps):
         x.set(0.0)
         r = s.Solve(b, x)
         ms.append(r.solve_ms)
-    out["fp64_cg"] = {"ms": round(min(ms), 3), "iters": r.numIter, "relres_fp64": relres(pr, x.numpy())}
+    x64 = x.numpy()
+    out["fp64_cg"] = {"ms": round(min(ms), 3), "iters": r.numIter, "relres_fp64": relres(pr, x64)}
     ir = fx.IR(ctx, A, 1.0, fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
                fx.params(maxIter=20000, relTol=1e-4, absTol=0.0))
     ms = []
@@ -131,8 +168,12 @@ def leg_powerlaw(fx, P, ctx, flush, sink, warmup, reps):
         x.set(0.0)
         rr, inner = ir.Solve(b, x)
         ms.append(rr.solve_ms)
+    xir = x.numpy()
     out["ir_cg"] = {"ms": round(min(ms), 3), "outer": rr.numIter, "inner": inner,
-                    "relres_fp64": relres(pr, x.numpy())}
+                    "relres_fp64": relres(pr, xir),
+                    "err_vs_fp64_cg": float(np.linalg.norm(xir - x64) / np.linalg.norm(x64))}
+    out["spmv_f64"]["traffic"] = profiled_traffic("k_spmv_f64_powerlaw_1048576_traffic.json",
+                                                  A.index_bytes, nnz)
     out["speedup_ir_vs_fp64"] = round(out["fp64_cg"]["ms"] / out["ir_cg"]["ms"], 3)
     log(f"powerlaw: SpMV f64 {out['spmv_f64']['GBs_flushed']:.0f} GB/s, f32 "
         f"{out['spmv_f32']['GBs_flushed']:.0f} GB/s; FP64 CG {out['fp64_cg']['ms']:.2f} ms "
@@ -178,6 +219,8 @@ def main() -> int:
     ap.add_argument("--spmv-warmup", type=int, default=20)
     ap.add_argument("--spmv-only", action="store_true",
                     help="standalone SpMV and a 256 MiB copy only (for profiler counter runs)")
+    ap.add_argument("--spmv-workload", choices=("poisson3d_128", "powerlaw_1048576"),
+                    default="poisson3d_128", help="the matrix of --spmv-only")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the convection-diffusion and power-law legs")
@@ -200,7 +243,7 @@ def main() -> int:
     ctx = fx.Context(0, stream=stream.cuda_stream)
 
     t0 = time.time()
-    pr = P.config(2)
+    pr = P.config(4 if args.spmv_only and args.spmv_workload == "powerlaw_1048576" else 2)
     A64 = pr.A
     n, nnz = A64.nrow, A64.nnz
     log(f"generated {pr.name}: n={n} nnz={nnz} in {time.time() - t0:.1f}s")
@@ -270,6 +313,7 @@ def main() -> int:
         rcg = scg.Solve(b, x)
         cg_ms.append(rcg.solve_ms)
     xr = x.numpy()
+    x_fp64 = xr.copy()
     cg_relres = float(np.linalg.norm(pr.b - A64.matvec(xr)) / np.linalg.norm(pr.b))
     cg_iter_bytes = 12 * nnz + 4 * (n + 1) + 13 * 8 * n
     cg = {"ms": min(cg_ms), "iters": rcg.numIter, "relres_fp64": cg_relres,
@@ -295,19 +339,30 @@ def main() -> int:
     wall_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
     xr = x.numpy()
     ir_relres = float(np.linalg.norm(pr.b - A64.matvec(xr)) / np.linalg.norm(pr.b))
+    ir_err = {"vs_fp64_cg": float(np.linalg.norm(xr - x_fp64) / np.linalg.norm(x_fp64)),
+              "vs_xstar": float(np.linalg.norm(xr - pr.xstar) / np.linalg.norm(pr.xstar)),
+              "fp64_cg_vs_xstar": float(np.linalg.norm(x_fp64 - pr.xstar)
+                                        / np.linalg.norm(pr.xstar))}
     ir_ms = statistics.median(step_ms)
     log(f"IR: {ir_ms:.2f} ms/solve ({outer_its} outer, {inner_its} inner), relres {ir_relres:.2e}")
 
     # ---- CPU baseline: the compiled reference's Jacobi-PCG, full solve -----
+    # The serial build (the reference's default, OpenMP off) is the baseline;
+    # its OpenMP build (USE_OPENMP: only VEC Dot / AXPBY / SetValues run in
+    # parallel) is reported beside it on the host's available cores.
     cpu = None
     if not args.no_cpu:
         try:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            from oracle import CG as RCG, Matrix, Reference
-            Rf = Reference()
+            from oracle import CG as RCG, REF_OMP_SO, Matrix, Reference
             M = Matrix.of(A64)
-            out = Rf.solve(RCG, M, pr.b, np.zeros(n), pc="jacobi", weight=1.0, maxIter=20000,
-                           relTol=1e-8, absTol=0.0)
+
+            def ref_run(lib):
+                out = lib.solve(RCG, M, pr.b, np.zeros(n), pc="jacobi", weight=1.0,
+                                maxIter=20000, relTol=1e-8, absTol=0.0)
+                return out, lib.threads()
+
+            out, _ = ref_run(Reference())
             cpu = {"value": round(out["solve_ms"], 1), "unit": "ms", "cores": 1,
                    "kind": "reference",
                    "sample": (f"reference CG::Solve with a Jacobi preconditioner (oracle/_ref, "
@@ -315,22 +370,21 @@ def main() -> int:
                               f"system to rtol 1e-8: {out['numIter']} iterations, "
                               f"steady_clock around Solve only; retcode {out['retcode']}"),
                    "iters": out["numIter"], "speedup_gpu_fp64_cg": round(out["solve_ms"] / cg["ms"], 1)}
+            cpu.update(cpu_info())
             log(f"CPU reference: {out['solve_ms']:.0f} ms, {out['numIter']} iters")
+            if Reference.available(REF_OMP_SO):
+                o2, th = ref_run(Reference(REF_OMP_SO))
+                cpu["openmp"] = {"value": round(o2["solve_ms"], 1), "unit": "ms", "threads": th,
+                                 "iters": o2["numIter"],
+                                 "sample": "the same solve, reference built with -fopenmp"}
+                log(f"CPU reference, OpenMP {th} threads: {o2['solve_ms']:.0f} ms, "
+                    f"{o2['numIter']} iters")
         except Exception as e:  # reference library absent on this host
             cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "reference",
                    "sample": f"unavailable: {e}"}
 
-    # HBM bytes per launch of the f64 SpMV, from the committed profiler
-    # counter summary (FETCH_SIZE / WRITE_SIZE runs of `bench.py --spmv-only`,
-    # scaled by a copy of known size); null when it was taken on another
-    # column-index format
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_spmv_f64_poisson3d_128_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tr = json.load(fh)
-        if tr.get("index_bytes") == A.index_bytes and tr.get("nnz") == nnz:
-            traffic = tr["traffic_bytes"]
+    # HBM bytes per launch of the f64 SpMV (profiler counters, committed)
+    traffic = profiled_traffic("k_spmv_f64_poisson3d_128_traffic.json", A.index_bytes, nnz)
 
     f64 = res_spmv["f64"]
     line = {
@@ -351,6 +405,7 @@ def main() -> int:
                    "inner_rtol": args.inner_rtol, "outer_iters": outer_its,
                    "inner_iters": inner_its},
         "relres_fp64": ir_relres,
+        "error_fp64": ir_err,
         "fp64_cg": cg,
         "speedup_ir_vs_fp64": round(cg["ms"] / ir_ms, 3),
         "spmv": res_spmv,
@@ -360,7 +415,12 @@ def main() -> int:
                      "achieved": round(f64["GBs_flushed"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(f64["GBs_flushed"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": f64["bytes"],
-                     "index_bytes": A.index_bytes},
+                     "index_bytes": A.index_bytes,
+                     # the HBM bytes the stored format actually moves, per second
+                     "traffic_GBs": (round(traffic / f64["ms_mean_flushed"] / 1e6, 1)
+                                     if traffic else None),
+                     "traffic_frac": (round(traffic / f64["ms_mean_flushed"] / 1e6
+                                            / HBM_PEAK_GBS, 4) if traffic else None)},
         "cpu_baseline": cpu,
         "other_configs": extra,
     }

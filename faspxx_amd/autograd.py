@@ -13,11 +13,39 @@ adjoint solve A^T lam = g with the same solver over the CSR transpose
 """
 from __future__ import annotations
 
+import contextlib
 import dataclasses
 
 import torch
 
 from . import fx
+
+
+def _ctx_stream(ctx: fx.Context) -> torch.cuda.Stream:
+    """The library context's HIP stream as a torch stream object."""
+    s = ctx.stream
+    if s == 0:
+        return torch.cuda.default_stream(ctx.device)
+    return torch.cuda.ExternalStream(s, device=torch.device("cuda", ctx.device))
+
+
+@contextlib.contextmanager
+def _ordered(ctx: fx.Context):
+    """Order library work on the context's stream after what torch's current
+    stream has queued (the inputs written there), and torch's current stream
+    after the library work (the outputs it returns).  Tensors used inside
+    must be allocated before entering, so that the wait covers their
+    initialisation; freeing them afterwards is ordered by the second wait."""
+    cur = torch.cuda.current_stream(ctx.device)
+    fs = _ctx_stream(ctx)
+    same = fs.cuda_stream == cur.cuda_stream
+    if not same:
+        fs.wait_stream(cur)
+    try:
+        yield
+    finally:
+        if not same:
+            cur.wait_stream(fs)
 
 
 @dataclasses.dataclass
@@ -60,17 +88,19 @@ class SparsePattern:
                                  restart=o.restart))
         b = b.contiguous()
         x = torch.zeros_like(b)
-        vb = fx.Vec.wrap(self.ctx, b.data_ptr(), self.n)
-        vx = fx.Vec.wrap(self.ctx, x.data_ptr(), self.n)
-        res = s.Solve(vb, vx)
-        vb.close()
-        vx.close()
+        with _ordered(self.ctx):
+            vb = fx.Vec.wrap(self.ctx, b.data_ptr(), self.n)
+            vx = fx.Vec.wrap(self.ctx, x.data_ptr(), self.n)
+            res = s.Solve(vb, vx)
+            vb.close()
+            vx.close()
         return x, res
 
     def set_values(self, values: torch.Tensor):
         values = values.contiguous()
-        self.A.set_values_dev(values.data_ptr())
-        self.A.transpose_update(self.At)
+        with _ordered(self.ctx):
+            self.A.set_values_dev(values.data_ptr())
+            self.A.transpose_update(self.At)
         # the Jacobi diagonals depend on the values: rebuild the solvers
         self._solvers.clear()
 
@@ -83,11 +113,12 @@ class SparsePattern:
 
     def pattern_grad(self, lam: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         out = torch.empty(self.nnz, dtype=torch.float64, device=lam.device)
-        vl = fx.Vec.wrap(self.ctx, lam.data_ptr(), self.n)
-        vx = fx.Vec.wrap(self.ctx, x.data_ptr(), self.n)
-        self.A.pattern_grad(vl, vx, out.data_ptr())
-        vl.close()
-        vx.close()
+        with _ordered(self.ctx):
+            vl = fx.Vec.wrap(self.ctx, lam.data_ptr(), self.n)
+            vx = fx.Vec.wrap(self.ctx, x.data_ptr(), self.n)
+            self.A.pattern_grad(vl, vx, out.data_ptr())
+            vl.close()
+            vx.close()
         return out
 
 

@@ -1,5 +1,6 @@
 // fx_core.hip -- context, device vectors and the VEC operations
 // (src/VEC.cxx) of the gfx950 iterative-solve library.
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 
@@ -18,6 +19,11 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 void clear_error() { g_err[0] = 0; }
+
+uint64_t next_handle_gen() {
+  static std::atomic<uint64_t> gen{0};
+  return gen.fetch_add(1, std::memory_order_relaxed) + 1;
+}
 
 // ---------------------------------------------------------------------------
 // Elementwise kernels.  Each thread handles elements i = tid, tid + stride, ...

@@ -1197,8 +1197,11 @@ struct IrInner {
     return 0;
   }
 
+  // The legacy null stream cannot be captured: replay by direct launches.
+  bool use_graph() const { return st != nullptr; }
+
   int build_graph() {
-    if (W.chunk[0]) return 0;
+    if (W.chunk[0] || !use_graph()) return 0;
     for (int slot = 0; slot < 2; ++slot) {
       hipGraph_t g = nullptr;
       FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1216,7 +1219,12 @@ struct IrInner {
   }
 
   int launch_chunk(int slot) {
-    FX_HIP(hipGraphLaunch(W.chunk[slot], st));
+    if (use_graph()) {
+      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
+    } else {
+      for (int c = 0; c < kChunk; ++c)
+        FX_TRY(launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr));
+    }
     FX_HIP(hipEventRecord(W.ev[slot], st));
     return 0;
   }
@@ -1262,6 +1270,8 @@ bool fast_eligible(fx_solver S) {
 }
 
 int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
+  // a workspace is sized for one matrix: never run it on another size
+  if (S->fast && S->fast->n != S->A->nrow) S->release_fast();
   if (!S->fast) {
     S->fast = new FastWs();
     const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacState)
@@ -1535,6 +1545,7 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         o.owned.push_back(vb);
         Ops::chk(fx_vec_wrap(ctx, n, FX_F32, ir->d32, &vx));
         o.owned.push_back(vx);
+        ir->inner_s->restart_cur = 0;  // every inner solve starts as set up
         (void)ops_gmres(ir->inner_s, o, vb, vx, &res_in);
       }
       inner_total += res_in.numIter;

@@ -566,7 +566,7 @@ struct GmRun {
     const fx_params& PP = S->prm;
     const uint32_t n = W.n;
     uint32_t maxR, minR, restart;
-    clamp_restart(PP, maxR, minR, restart);
+    solve_restart(S, maxR, minR, restart);
     VT* x = (VT*)W.x;
     FX_HIP(hipMemcpyAsync(x, x_user, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
     if (flex) {  // the stale Z of earlier solves: their max-norms
@@ -611,6 +611,7 @@ struct GmRun {
       ri = rj;
       adapt_restart(cr, flex, maxR, minR, restart);  // FGMRES: <= / >= (src/FGMRES.cxx:251)
     }
+    S->restart_cur = restart;
     res->numIter = s.numIter;
     if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
       res->norm2 = left ? s.rj : resAbs;  // the norm of tmp (left) or V[0] (right), :269-272
@@ -661,7 +662,7 @@ int gmres_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result* re
   if (flex) {
     // FGMRES keeps Z in the solver (shared with the per-operation path, so
     // the stale Z carries over between solves as in the reference)
-    if (S->Z.size() != maxR + 1) {
+    if (S->Z.size() != maxR + 1 || S->Z[0]->n != W.n) {
       for (fx_vec z : S->Z) fx_vec_destroy(z);
       S->Z.assign(maxR + 1, nullptr);
       for (auto& z : S->Z) FX_TRY(fx_vec_create(S->ctx, W.n, FX_F64, &z));

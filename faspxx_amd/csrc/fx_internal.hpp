@@ -30,6 +30,12 @@ constexpr uint32_t kMaxStag = 20;     // include/Faspxx.hxx:44
 void set_error(const char* fmt, ...);
 void clear_error();
 
+// A process-wide increasing id, given to each matrix and preconditioner
+// handle when it is made.  A solver remembers the ids of what it was set up
+// with: a handle freed and made again may come back at the same address,
+// and its device arrays, sizes and captured kernel arguments are then new.
+uint64_t next_handle_gen();
+
 }  // namespace fx
 
 #define FX_HIP(call)                                                                 \
@@ -93,6 +99,7 @@ struct fx_vec_s {
 // reduced by the whole workgroup).
 struct fx_csr_s {
   fx_ctx ctx = nullptr;
+  uint64_t gen = fx::next_handle_gen();
   uint32_t nrow = 0, mcol = 0, nnz = 0;
   bool has_values = false;
   uint32_t* rp = nullptr;
@@ -127,6 +134,7 @@ struct fx_csr_s {
 
 struct fx_pc_s {
   fx_ctx ctx = nullptr;
+  uint64_t gen = fx::next_handle_gen();
   int kind = 0;  // 0 identity, 1 jacobi, 2 symmetric multicolor Gauss-Seidel
   fx_mcgs mcgs = nullptr;  // kind 2
   fx_csr A = nullptr;

@@ -390,6 +390,11 @@ void clamp_restart(const fx_params& P, uint32_t& maxR, uint32_t& minR, uint32_t&
 // Variable restart after a cycle: GMRES strict (src/GMRES.cxx:256-265),
 // FGMRES inclusive (src/FGMRES.cxx:247-256); unsigned subtraction as in the
 // reference.
+void solve_restart(fx_solver S, uint32_t& maxR, uint32_t& minR, uint32_t& restart) {
+  clamp_restart(S->prm, maxR, minR, restart);
+  if (S->restart_cur) restart = S->restart_cur < maxR ? S->restart_cur : maxR;
+}
+
 void adapt_restart(double cr, bool inclusive, uint32_t maxR, uint32_t minR, uint32_t& restart) {
   const double max_cr = 0.99026806874157040, min_cr = 0.17364817766693041;
   const uint32_t dec = 3;
@@ -409,7 +414,7 @@ int ops_gmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
   const fx_params& P = S->prm;
   const bool left = P.leftPrecond != 0;
   uint32_t maxR, minR, restart;
-  clamp_restart(P, maxR, minR, restart);
+  solve_restart(S, maxR, minR, restart);
   int err = FX_SUCCESS;
   double t, ri, rj, cr;
   double resAbs = 1.0, resRel = 1.0, denAbs = 1.0, ratio = 0.0, resAbsOld = 1.0;
@@ -508,6 +513,7 @@ int ops_gmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
     ri = rj;
     adapt_restart(cr, false, maxR, minR, restart);
   }
+  S->restart_cur = restart;
   res->numIter = numIter;
   if (!(numIter == P.minIter && numIter == P.maxIter)) {
     fx_vec rv = left ? tmp : V[0];
@@ -523,7 +529,7 @@ int ops_gmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
 int ops_fgmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
   const fx_params& P = S->prm;
   uint32_t maxR, minR, restart;
-  clamp_restart(P, maxR, minR, restart);
+  solve_restart(S, maxR, minR, restart);
   int err = FX_SUCCESS;
   double t, ri, rj, cr;
   double resAbs = 1.0, resRel = 1.0, denAbs = 1.0, ratio = 0.0, resAbsOld = 1.0;
@@ -532,7 +538,7 @@ int ops_fgmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
   G.init(maxR);
   std::vector<fx_vec> V(maxR + 1);
   for (auto& v : V) v = o.vec();
-  if (S->Z.size() != maxR + 1) {
+  if (S->Z.size() != maxR + 1 || S->Z[0]->n != o.n || S->Z[0]->dtype != o.dt) {
     for (fx_vec z : S->Z) fx_vec_destroy(z);
     S->Z.assign(maxR + 1, nullptr);
     for (auto& z : S->Z) Ops::chk(fx_vec_create(S->ctx, o.n, o.dt, &z));
@@ -597,6 +603,7 @@ int ops_fgmres(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res) {
     ri = rj;
     adapt_restart(cr, true, maxR, minR, restart);
   }
+  S->restart_cur = restart;
   res->numIter = numIter;
   if (!(numIter == P.minIter && numIter == P.maxIter)) {
     res->norm2 = o.norm2(V[0]);
@@ -758,20 +765,55 @@ int fx_solver_create(fx_ctx ctx, int type, fx_solver* out) {
   return 0;
 }
 
-int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm) {
+}  // extern "C"
+
+namespace {
+
+// Bind a matrix, a preconditioner and parameters to a solver.  setup: the
+// reference's Setup, which resets the GMRES / FGMRES restart length
+// (src/GMRES.cxx:44,75-79); otherwise the reference's Solve reading its A,
+// pcd and params as they are now, which keeps it.
+int solver_bind(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm, bool setup) {
   clear_error();
   FX_CHECK(s && A && pc, FX_ERROR_INPUT_PAR, "null argument");
   FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "solvers need a square matrix");
   if (s->type == FX_SOLVER_JACOBI)
     FX_CHECK(pc->kind == 1, FX_ERROR_SOLVER_PCD_TYPE, "Jacobi solver needs a Jacobi pc");
   const fx_params np = prm ? *prm : s->prm;
-  // keep the fused-path workspace (and its replay graph) when nothing changed
-  const bool same = s->A == A && s->pc == pc && std::memcmp(&np, &s->prm, sizeof(fx_params)) == 0;
+  // Keep the fused-path workspace (and its replay graphs) only when the
+  // same matrix and preconditioner handles are bound again with the same
+  // parameters.  Handles are compared by their ids, not their addresses: a
+  // matrix freed and made again may reuse the address of the old one.
+  const bool same = solver_reuse_ok(s->A, s->A_gen, s->pc, s->pc_gen, A, pc) &&
+                    std::memcmp(&np, &s->prm, sizeof(fx_params)) == 0;
   s->A = A;
   s->op = nullptr;
   s->pc = pc;
+  s->A_gen = A->gen;
+  s->pc_gen = pc->gen;
   s->prm = np;
+  if (setup) s->restart_cur = 0;
   if (!same) s->release_fast();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm) {
+  return solver_bind(s, A, pc, prm, true);
+}
+
+int fx_solver_update(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm) {
+  return solver_bind(s, A, pc, prm, false);
+}
+
+int fx_solver_clean(fx_solver s) {
+  clear_error();
+  FX_CHECK(s, FX_ERROR_INPUT_PAR, "null solver");
+  s->restart_cur = 0;
+  for (fx_vec z : s->Z) FX_TRY(fx_vec_set(z, 0.0));
   return 0;
 }
 
@@ -794,7 +836,11 @@ int fx_op_destroy(fx_op op) {
   return 0;
 }
 
-int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm) {
+}  // extern "C"
+
+namespace {
+
+int solver_bind_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm, bool setup) {
   clear_error();
   FX_CHECK(s && A && pc, FX_ERROR_INPUT_PAR, "null argument");
   FX_CHECK(A->nrow == A->mcol, FX_ERROR_MAT_SIZE, "solvers need a square operator");
@@ -804,8 +850,23 @@ int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm) {
   s->A = nullptr;
   s->op = A;
   s->pc = pc;
+  s->A_gen = 0;
+  s->pc_gen = pc->gen;
   s->prm = prm ? *prm : s->prm;
+  if (setup) s->restart_cur = 0;
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm) {
+  return solver_bind_op(s, A, pc, prm, true);
+}
+
+int fx_solver_update_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm) {
+  return solver_bind_op(s, A, pc, prm, false);
 }
 
 int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
@@ -829,7 +890,8 @@ int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
   FX_HIP(hipEventRecord(e0, s->ctx->stream));
   int rc = 0;
   try {
-    if (!s->op && fast_eligible(s)) {
+    s->last_path = (!s->op && fast_eligible(s)) ? FX_PATH_FUSED : FX_PATH_OPS;
+    if (s->last_path == FX_PATH_FUSED) {
       rc = fast_solve(s, b, x, res);
     } else {
       Ops o = s->op ? Ops(s->ctx, s->op, s->pc, FX_F64) : Ops(s->ctx, s->A, s->pc, FX_F64);
@@ -865,6 +927,13 @@ int fx_solver_history(fx_solver s, double* out, uint32_t cap, uint32_t* count) {
     const uint32_t m = c < cap ? c : cap;
     for (uint32_t i = 0; i < m; ++i) out[i] = i < s->hist_host.size() ? s->hist_host[i] : 0.0;
   }
+  return 0;
+}
+
+int fx_solver_last_path(fx_solver s, int* path) {
+  clear_error();
+  FX_CHECK(s && path, FX_ERROR_INPUT_PAR, "null argument");
+  *path = s->last_path;
   return 0;
 }
 

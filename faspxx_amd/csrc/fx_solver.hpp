@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "fx_internal.hpp"
+#include "fx_reuse.hpp"
 
 namespace fx {
 
@@ -24,9 +25,16 @@ struct fx_solver_s {
   fx_csr A = nullptr;
   fx_op op = nullptr;  // set instead of A by fx_solver_setup_op
   fx_pc pc = nullptr;
+  uint64_t A_gen = 0, pc_gen = 0;  // the handles' ids at setup
   fx_params prm{};
   std::vector<double> hist_host;
   uint32_t hist_count = 0;
+  int last_path = FX_PATH_OPS;  // the form the last Solve ran
+  // GMRES / FGMRES: the restart length the last Solve ended with (0: as set
+  // up).  The reference keeps it in the solver: Setup resets it
+  // (src/GMRES.cxx:44,75-79), Solve adapts it per cycle (:255-265) and the
+  // next Solve starts from there.
+  uint32_t restart_cur = 0;
   std::vector<fx_vec> Z;  // FGMRES preconditioned basis, kept across calls
   fx::FastWs* fast = nullptr;
   void record(uint32_t it, double v);
@@ -85,6 +93,9 @@ struct Givens {
 };
 
 void clamp_restart(const fx_params& P, uint32_t& maxR, uint32_t& minR, uint32_t& restart);
+// The restart bounds and the length a Solve starts with (the one the last
+// Solve on S ended with, if any since setup).
+void solve_restart(fx_solver S, uint32_t& maxR, uint32_t& minR, uint32_t& restart);
 void adapt_restart(double cr, bool inclusive, uint32_t maxR, uint32_t minR, uint32_t& restart);
 
 int ops_cg(fx_solver S, Ops& o, fx_vec b, fx_vec x, fx_result* res);

@@ -132,14 +132,25 @@ __global__ void k_scaled_copy(const uint32_t* __restrict__ ci, const double* __r
   }
 }
 
-// values[diagPtr[j]] += a for the rows whose diagPtr names their diagonal
-// entry (MAT::Shift, src/MAT.cxx:359-364)
-__global__ void k_shift_diag(const uint32_t* __restrict__ ci, const uint32_t* __restrict__ dp,
-                             double* __restrict__ va, uint32_t nd, double a) {
+// MAT::Shift (src/MAT.cxx:359-364): values[diagPtr[j]] += a for every row
+// j, in the reference's loop over diagPtr.  A row without a diagonal entry
+// keeps diagPtr 0 (FormDiagPtr), so it adds a to values[0] once more.  Adds
+// that land on one entry each add the same a, so the entry goes through the
+// same sequence v, v + a, (v + a) + a, ... whichever row adds first: an
+// atomic compare-and-swap loop per row gives the reference's bits exactly.
+__global__ void k_shift_diag(const uint32_t* __restrict__ dp, double* __restrict__ va,
+                             uint32_t nrow, uint32_t nnz, double a) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= nd) return;
+  if (j >= nrow) return;
   const uint32_t d = dp[j];
-  if (ci[d] == j) va[d] += a;
+  if (d >= nnz) return;
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(va + d);
+  unsigned long long old = *p, seen;
+  do {
+    seen = old;
+    const double nv = __longlong_as_double((long long)seen) + a;
+    old = atomicCAS(p, seen, (unsigned long long)__double_as_longlong(nv));
+  } while (old != seen);
 }
 
 // ---- L R ------------------------------------------------------------------
@@ -389,9 +400,8 @@ int fx_csr_shift_diag(fx_ctx ctx, fx_csr A, double a) {
   clear_error();
   FX_CHECK(ctx && A, FX_ERROR_INPUT_PAR, "null argument");
   if (A->nnz == 0 || !A->has_values) return 0;  // structure only: src/MAT.cxx:361
-  const uint32_t nd = A->nrow < A->mcol ? A->nrow : A->mcol;
-  hipLaunchKernelGGL(k_shift_diag, dim3(row_blocks(nd)), dim3(kThreads), 0, ctx->stream, A->ci,
-                     A->dp, A->va, nd, a);
+  hipLaunchKernelGGL(k_shift_diag, dim3(row_blocks(A->nrow)), dim3(kThreads), 0, ctx->stream,
+                     A->dp, A->va, A->nrow, A->nnz, a);
   FX_HIP(hipGetLastError());
   return fx_csr_set_values_dev(A, A->va);
 }

@@ -146,6 +146,10 @@ _SIGS = {
     "fx_solver_setup": (C.c_int, [_P, _P, _P, C.POINTER(Params)]),
     "fx_solver_solve": (C.c_int, [_P, _P, _P, C.POINTER(Result)]),
     "fx_solver_history": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "fx_solver_last_path": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "fx_solver_update": (C.c_int, [_P, _P, _P, C.POINTER(Params)]),
+    "fx_solver_clean": (C.c_int, [_P]),
+    "fx_solver_update_op": (C.c_int, [_P, _P, _P, C.POINTER(Params)]),
     "fx_solver_destroy": (C.c_int, [_P]),
     "fx_krylov": (C.c_int, [_P, _P, _P, _P, _P, C.c_char_p, C.POINTER(Params),
                             C.POINTER(Result)]),
@@ -639,6 +643,15 @@ class Solver:
         else:
             _chk(load().fx_solver_setup(self.h, A.h, pc.h, C.byref(prm)))
 
+    def Update(self, A: "Csr", pc: Precond, prm: Params):
+        """Bind A, pc and params again without a new Setup: GMRES / FGMRES
+        keep their restart length (and FGMRES its Z)."""
+        self._keep = (A, pc)
+        _chk(load().fx_solver_update(self.h, A.h, pc.h, C.byref(prm)))
+
+    def Clean(self):
+        _chk(load().fx_solver_clean(self.h))
+
     def Solve(self, b: Vec, x: Vec) -> Result:
         """Returns the Result; retcode carries FaspRetCode (negative codes such
         as ERROR_SOLVER_MAXIT are results, not exceptions)."""
@@ -647,6 +660,14 @@ class Solver:
         if rc in (-90, -12, -16, -20):
             _chk(rc)
         return res
+
+    @property
+    def last_path(self) -> str:
+        """'fused' (graph-replayed fused kernels) or 'ops' (per-operation
+        form): the form the last Solve ran."""
+        p = C.c_int()
+        _chk(load().fx_solver_last_path(self.h, C.byref(p)))
+        return "fused" if p.value == 1 else "ops"
 
     def history(self) -> np.ndarray:
         cnt = C.c_uint32()

@@ -672,7 +672,15 @@ FaspRetCode KrylovSOL::Setup(const LOP& A) {
   this->A = &A;
   mat = dynamic_cast<const MAT*>(&A);
   lop = mat ? nullptr : &A;  // any other LOP: its Apply is called back
+  setup_pending = true;  // the next Solve sets the device solver up afresh
   return SUCCESS;
+}
+
+// GMRES::Clean / FGMRES::Clean (src/GMRES.cxx:90-110, src/FGMRES.cxx:83-105):
+// restart length back to the set-up value, FGMRES's Z zeroed.  CG and
+// BiCGStab only zero work vectors every Solve overwrites.
+void KrylovSOL::Clean() {
+  if (solver) check(fx_solver_clean(solver), "fx_solver_clean");
 }
 
 FaspRetCode KrylovSOL::Solve(const VEC& b, VEC& x) {
@@ -692,14 +700,23 @@ FaspRetCode KrylovSOL::Solve(const VEC& b, VEC& x) {
   prm.leftPrecond = rightPrecond ? 0 : 1;
   int rc = 0;
   if (mat) {
-    rc = fx_solver_setup(solver, mat->Device(), p, &prm);
+    // the reference's Solve reads A, pcd and params as they are now; only a
+    // Setup since the last Solve resets the GMRES / FGMRES restart length
+    rc = setup_pending ? fx_solver_setup(solver, mat->Device(), p, &prm)
+                       : fx_solver_update(solver, mat->Device(), p, &prm);
+    setup_pending = false;
   } else {
-    if (op) fx_op_destroy(op);
-    op = nullptr;
-    check(fx_op_create(Context(), lop->GetRowSize(), lop->GetColSize(), lop_apply,
-                       const_cast<LOP*>(lop), &op),
-          "fx_op_create");
-    rc = fx_solver_setup_op(solver, op, p, &prm);
+    if (setup_pending || !op) {
+      if (op) fx_op_destroy(op);
+      op = nullptr;
+      check(fx_op_create(Context(), lop->GetRowSize(), lop->GetColSize(), lop_apply,
+                         const_cast<LOP*>(lop), &op),
+            "fx_op_create");
+      rc = fx_solver_setup_op(solver, op, p, &prm);
+    } else {
+      rc = fx_solver_update_op(solver, op, p, &prm);
+    }
+    setup_pending = false;
   }
   fx_result res{};
   if (!rc) rc = fx_solver_solve(solver, b.Device(), x.DeviceOut(), &res);

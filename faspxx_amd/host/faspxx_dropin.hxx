@@ -387,10 +387,11 @@ class KrylovSOL : public SOL {
   ~KrylovSOL() override;
   FaspRetCode Setup(const LOP& A) override;
   FaspRetCode Solve(const VEC& b, VEC& x) override;
-  void Clean() override {}
+  void Clean() override;
 
  protected:
   SOLType kind;
+  bool setup_pending = true;  // Setup ran since the last Solve
   const MAT* mat = nullptr;
   const LOP* lop = nullptr;  // an operator that is not a MAT: its Apply is called back
   fx_op op = nullptr;

@@ -233,9 +233,9 @@ int fx_csr_add(fx_ctx ctx, double a, fx_csr A, double b, fx_csr B, fx_csr* C);
  * product; more than 2^31 candidate products is FX_ERROR_ALLOC_MEM.
  *   MAT::Mult, src/MAT.cxx:722-800 */
 int fx_csr_mult(fx_ctx ctx, fx_csr L, fx_csr R, fx_csr* C);
-/* values[diagPtr[j]] += a in place, for every row whose diagPtr names its
- * diagonal entry (the reference also adds to the entry a stale diagPtr names
- * in a row without one).   MAT::Shift, src/MAT.cxx:359-364 */
+/* values[diagPtr[j]] += a in place for every row j, as the reference's loop
+ * over diagPtr: a row without a diagonal entry has diagPtr 0 and adds a to
+ * values[0] (once per such row).   MAT::Shift, src/MAT.cxx:359-364 */
 int fx_csr_shift_diag(fx_ctx ctx, fx_csr A, double a);
 
 /* Refresh the values of At (made by fx_csr_transpose from A) from A's
@@ -281,8 +281,18 @@ int fx_pc_create_mcgs(fx_ctx ctx, fx_csr A, double weight, uint32_t sweeps, fx_p
 int fx_solver_create(fx_ctx ctx, int type, fx_solver* out);
 /* SetupPCD(pc) + setters + Setup(A): src/SOL.cxx:249, src/CG.cxx:16-40 etc.
  * For FX_SOLVER_JACOBI, pc must be a Jacobi preconditioner; its weight is
- * the solver's weight and its iteration count is ignored. */
+ * the solver's weight and its iteration count is ignored.  As Setup does,
+ * it resets the GMRES / FGMRES restart length (src/GMRES.cxx:44,75-79). */
 int fx_solver_setup(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm);
+/* A, pc and params as the reference's Solve finds them without a new Setup
+ * (SOL keeps pointers to A and pcd, src/SOL.hxx:28-29, and setters change
+ * params): bound again, while GMRES / FGMRES keep the restart length the
+ * last Solve ended with (the member `restart`, adapted per cycle,
+ * src/GMRES.cxx:255-265) and FGMRES its Z (src/FGMRES.cxx:158-159). */
+int fx_solver_update(fx_solver s, fx_csr A, fx_pc pc, const fx_params* prm);
+/* Clean(): the GMRES / FGMRES restart length back to the set-up value and
+ * FGMRES's kept Z zeroed (src/GMRES.cxx:90-110, src/FGMRES.cxx:83-105). */
+int fx_solver_clean(fx_solver s);
 /* A linear operator given by its action: a LOP subclass with its own Apply
  * (src/LOP.hxx:22-66; the matrix-free pattern of example/TestMatFree.cxx).
  * The Krylov solvers (CG, BiCGStab, GMRES, FGMRES) accept it through
@@ -293,6 +303,8 @@ int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply,
                  fx_op* out);
 int fx_op_destroy(fx_op op);
 int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm);
+/* fx_solver_update for an operator (see fx_solver_update). */
+int fx_solver_update_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm);
 /* Solve(b, x): x is the initial guess on entry (CG::Solve, src/CG.cxx:53-249;
  * BiCGStab::Solve, src/BiCGStab.cxx:68-270; GMRES::Solve, src/GMRES.cxx:16-22;
  * FGMRES::Solve, src/FGMRES.cxx:108-271; Jacobi::Solve, src/Iter.cxx:56-112).
@@ -301,6 +313,13 @@ int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* result);
 /* Residual norms the reference prints per iteration (SOL::PrintInfo,
  * src/SOL.cxx:46-64), index = iteration number. */
 int fx_solver_history(fx_solver s, double* out, uint32_t cap, uint32_t* count);
+/* Which form ran the last Solve: FX_PATH_FUSED (graph-replayed fused
+ * kernels, scalars on the device) or FX_PATH_OPS (one device call per
+ * reference operation, scalars read back).  No reference counterpart (the
+ * reference has one CPU path). */
+#define FX_PATH_OPS 0
+#define FX_PATH_FUSED 1
+int fx_solver_last_path(fx_solver s, int* path);
 int fx_solver_destroy(fx_solver s);
 
 /* Free function Krylov(A, b, x, pcd, params), src/Krylov.cxx:16-53: name

@@ -98,6 +98,21 @@ struct orc_result {
   double norm2, normInf;
 };
 
+// Per-cycle trace of GMRES / FGMRES (test instrumentation; it changes no
+// arithmetic).  For cycle k < cap: len[k] inner iterations, planned[k] the
+// restart length the cycle started with (len[k] is shorter when the cycle
+// ends on convergence or maxIter), est[k] the Givens residual estimate
+// |g[len]| at the cycle's end, rtrue[k] the norm of the true residual
+// b - A x after the cycle's x update (for left-preconditioned GMRES, of its
+// preconditioned form).  count = cycles run.
+struct orc_cycles {
+  uint32_t cap, count;
+  uint32_t* len;
+  uint32_t* planned;
+  double* est;
+  double* rtrue;
+};
+
 // ---------------------------------------------------------------------------
 // VEC operations -- src/VEC.cxx.  Elementwise ops keep the reference's
 // expression shape; reductions keep its four strided accumulators.
@@ -401,8 +416,29 @@ struct Ctx {
   uint32_t n;
   double* hist;  // optional residual history (resAbs per reported iteration)
   uint32_t hist_cap;
+  orc_cycles* cyc = nullptr;  // optional GMRES / FGMRES cycle trace
+  double* vout = nullptr;     // optional: the basis V[0..maxR] of the last cycle
+  // optional GMRES / FGMRES restart length kept in the solver object (the
+  // reference's member `restart`): 0 on entry = as Setup leaves it; on exit
+  // the length the solve ended with
+  uint32_t* restart_io = nullptr;
   void rec(uint32_t it, double r) const {
     if (hist && it < hist_cap) hist[it] = r;
+  }
+  void cycle(uint32_t len, uint32_t planned, double est, double rtrue) const {
+    if (!cyc) return;
+    if (cyc->count < cyc->cap) {
+      cyc->len[cyc->count] = len;
+      cyc->planned[cyc->count] = planned;
+      cyc->est[cyc->count] = est;
+      cyc->rtrue[cyc->count] = rtrue;
+    }
+    ++cyc->count;
+  }
+  void basis(const std::vector<std::vector<double>>& V) const {
+    if (!vout) return;
+    for (size_t j = 0; j < V.size(); ++j)
+      std::memcpy(vout + j * n, V[j].data(), sizeof(double) * n);
   }
 };
 
@@ -728,6 +764,7 @@ int gmres(const Ctx& c, const double* b, double* x, orc_result* res) {
   const bool left = P.leftPrecond != 0;
   uint32_t maxR, minR, restart;
   clamp_restart(P, maxR, minR, restart);
+  if (c.restart_io && *c.restart_io) restart = *c.restart_io < maxR ? *c.restart_io : maxR;
   int err = RC_SUCCESS;
   double t, ri, rj, cr;
   double resAbs = 1.0, resRel = 1.0, denAbs = 1.0, ratio = 0.0, resAbsOld = 1.0;
@@ -811,6 +848,8 @@ int gmres(const Ctx& c, const double* b, double* x, orc_result* res) {
     } else {
       orc_axpy(n, 1.0, wk.data(), x);
     }
+    c.basis(V);
+    const double est = resAbs;
     if (numIter >= P.savIter && resAbs < resAbsOld) safe.assign(x, x + n);
     if (numIter > P.maxIter) {
       err = RC_SOLVER_MAXIT;
@@ -827,6 +866,7 @@ int gmres(const Ctx& c, const double* b, double* x, orc_result* res) {
       apply_pc(c, tmp.data(), V[0].data(), work);
       resAbs = orc_norm2(n, V[0].data());
     }
+    c.cycle(count, restart, est, resAbs);
     resRel = resAbs / denAbs;
     if ((resRel < P.relTol || resAbs < P.absTol) && numIter >= P.minIter) break;
 
@@ -834,6 +874,7 @@ int gmres(const Ctx& c, const double* b, double* x, orc_result* res) {
     ri = rj;
     adapt_restart(cr, false, maxR, minR, restart);
   }
+  if (c.restart_io) *c.restart_io = restart;
 
   res->numIter = numIter;
   if (!(numIter == P.minIter && numIter == P.maxIter)) {
@@ -854,6 +895,7 @@ int fgmres(const Ctx& c, const double* b, double* x, orc_result* res, std::vecto
   const orc_params& P = c.prm;
   uint32_t maxR, minR, restart;
   clamp_restart(P, maxR, minR, restart);
+  if (c.restart_io && *c.restart_io) restart = *c.restart_io < maxR ? *c.restart_io : maxR;
   int err = RC_SUCCESS;
   double t, ri, rj, cr;
   double resAbs = 1.0, resRel = 1.0, denAbs = 1.0, ratio = 0.0, resAbsOld = 1.0;
@@ -910,6 +952,8 @@ int fgmres(const Ctx& c, const double* b, double* x, orc_result* res, std::vecto
     orc_scale(n, g[count_1], wk.data());
     for (int j = (int)count - 2; j >= 0; --j) orc_axpby(n, 1.0, g[j], Z[j].data(), wk.data());
     orc_axpy(n, 1.0, wk.data(), x);
+    c.basis(V);
+    const double est = resAbs;
     if (numIter >= P.savIter && resAbs < resAbsOld) safe.assign(x, x + n);
     if (numIter > P.maxIter) {
       err = RC_SOLVER_MAXIT;
@@ -918,6 +962,7 @@ int fgmres(const Ctx& c, const double* b, double* x, orc_result* res, std::vecto
 
     true_residual(c, b, x, V[0].data());
     resAbs = rj = orc_norm2(n, V[0].data());
+    c.cycle(count, restart, est, resAbs);
     resRel = resAbs / denAbs;
     if ((resRel < P.relTol || resAbs < P.absTol) && numIter >= P.minIter) break;
 
@@ -925,6 +970,7 @@ int fgmres(const Ctx& c, const double* b, double* x, orc_result* res, std::vecto
     ri = rj;
     adapt_restart(cr, true, maxR, minR, restart);
   }
+  if (c.restart_io) *c.restart_io = restart;
 
   res->numIter = numIter;
   if (!(numIter == P.minIter && numIter == P.maxIter)) {
@@ -997,6 +1043,44 @@ int orc_solve(int kind, const orc_csr* A, const orc_pc* pc, const orc_params* pr
     case ORC_JACOBI: return jacobi(c, pc->dinv, b, x, res);
     default: res->retcode = RC_SOLVER_TYPE; return RC_SOLVER_TYPE;
   }
+}
+
+// GMRES / FGMRES with test instrumentation: the per-cycle trace (cyc, may be
+// null), the last cycle's basis V[0..maxR] after its x update (vout,
+// (maxR+1) n doubles, may be null), for FGMRES the solver's kept basis Z
+// (z_io, (maxR+1) n doubles: the stale Z on entry -- zeros for a freshly set
+// up solver -- and the kept Z on exit; null starts from zeros), and the
+// restart length the solver object keeps between Solve calls (restart_io:
+// 0 on entry = freshly set up; may be null).  With z_io and restart_io a
+// sequence of Solve calls on one reference solver object is replayed.
+int orc_krylov_traced(int kind, const orc_csr* A, const orc_pc* pc, const orc_params* prm,
+                      const double* b, double* x, orc_result* res, double* hist,
+                      uint32_t hist_cap, orc_cycles* cyc, double* vout, double* z_io,
+                      uint32_t* restart_io) {
+  Ctx c{A, pc, *prm, A->nrow, hist, hist_cap};
+  c.cyc = cyc;
+  c.vout = vout;
+  c.restart_io = restart_io;
+  if (cyc) cyc->count = 0;
+  res->retcode = 0;
+  res->numIter = 0;
+  res->norm2 = 1e60;
+  res->normInf = 1e60;
+  if (kind == ORC_GMRES) return gmres(c, b, x, res);
+  if (kind != ORC_FGMRES) {
+    res->retcode = RC_SOLVER_TYPE;
+    return RC_SOLVER_TYPE;
+  }
+  uint32_t maxR, minR, restart;
+  clamp_restart(*prm, maxR, minR, restart);
+  const uint32_t n = A->nrow;
+  std::vector<Vec> Z(maxR + 1, Vec(n, 0.0));
+  if (z_io)
+    for (uint32_t j = 0; j <= maxR; ++j) std::memcpy(Z[j].data(), z_io + (size_t)j * n, sizeof(double) * n);
+  const int rc = fgmres(c, b, x, res, Z);
+  if (z_io)
+    for (uint32_t j = 0; j <= maxR; ++j) std::memcpy(z_io + (size_t)j * n, Z[j].data(), sizeof(double) * n);
+  return rc;
 }
 
 }  // extern "C"
@@ -1272,6 +1356,19 @@ void orc_mat_mult(const orc_csr* L, const orc_csr* R, orc_mat_out* out) {
     }
   }
   emit(n, m, rp, ci, values ? &va : nullptr, out);
+}
+
+// MAT::Shift, src/MAT.cxx:359-364: for every row j in order,
+// values[diagPtr[j]] += a (a structure-only matrix is left alone).  A row
+// without a diagonal entry has diagPtr 0 from FormDiagPtr and adds to
+// values[0].  va is updated in place; diagPtr entries past nnz are skipped
+// (the reference would index out of bounds there).
+void orc_mat_shift(const orc_csr* A, double a, double* va) {
+  if (!A->va || A->nnz == 0) return;
+  for (uint32_t j = 0; j < A->nrow; ++j) {
+    const uint32_t d = A->dp[j];
+    if (d < A->nnz) va[d] += a;
+  }
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libfaspxx_ref.so"
+REF_OMP_SO = HERE / "_ref" / "libfaspxx_ref_omp.so"
 
 CG, BICGSTAB, GMRES, FGMRES, JACOBI = 1, 2, 4, 5, 11
 
@@ -69,6 +70,11 @@ def _take(o, with_dp):
     if with_dp:
         dp = np.ctypeslib.as_array(o.dp, (n,)).copy() if n else np.zeros(0, np.uint32)
     return rp, ci, va, dp
+
+
+class _Cycles(C.Structure):
+    _fields_ = [("cap", C.c_uint32), ("count", C.c_uint32), ("len", C.c_void_p),
+                ("planned", C.c_void_p), ("est", C.c_void_p), ("rtrue", C.c_void_p)]
 
 
 class _Result(C.Structure):
@@ -148,6 +154,10 @@ class Oracle:
             "orc_mat_tri": (None, [cs, C.c_int, C.POINTER(_MatOut)]),
             "orc_mat_add": (C.c_int, [d, cs, d, cs, C.POINTER(_MatOut)]),
             "orc_mat_mult": (None, [cs, cs, C.POINTER(_MatOut)]),
+            "orc_mat_shift": (None, [cs, d, v]),
+            "orc_krylov_traced": (C.c_int, [C.c_int, cs, C.POINTER(_Pc), C.POINTER(_Params), v,
+                                            v, C.POINTER(_Result), v, u, C.POINTER(_Cycles), v,
+                                            v, C.POINTER(C.c_uint32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -263,6 +273,52 @@ class Oracle:
             out["hist"] = hist
         return out
 
+    def krylov_traced(self, kind, M: Matrix, b, x0, pc="identity", weight=1.0, z=None,
+                      restart_in=0, want_basis=False, cycle_cap=100000, hist_cap=0, **prm):
+        """GMRES or FGMRES with the oracle's test instrumentation.  Returns
+        the solve() dict plus "cycles" (per cycle: inner iterations, the
+        Givens estimate at its end, the true residual norm after its x
+        update), "planned" (per cycle, the restart length it started with),
+        "V" (the last cycle's basis, when want_basis), "restart"
+        (the restart length the solver object keeps for its next Solve;
+        `restart_in` is the one it starts with, 0 = freshly set up) and, for
+        FGMRES, "Z" (the kept basis on exit; `z` is the stale basis on entry,
+        (maxR+1) x n, zeros when None)."""
+        P = dict(DEFAULTS)
+        P.update(prm)
+        params = _Params(**P)
+        maxR = min(P["maxRestart"], P["maxIter"])
+        n = M.nrow
+        b = _f64(b)
+        x = _f64(x0).copy()
+        dinv = self.jacobi_dinv(M, weight) if pc == "jacobi" else None
+        pcs = _Pc(1 if dinv is not None else 0, 1, _p(dinv), None, 0, weight)
+        res = _Result()
+        ln = np.zeros(cycle_cap, np.uint32)
+        planned = np.zeros(cycle_cap, np.uint32)
+        est = np.zeros(cycle_cap)
+        rtrue = np.zeros(cycle_cap)
+        cyc = _Cycles(cycle_cap, 0, _p(ln), _p(planned), _p(est), _p(rtrue))
+        V = np.zeros((maxR + 1, n)) if want_basis else None
+        Z = None
+        if kind == FGMRES:
+            Z = np.zeros((maxR + 1, n)) if z is None else np.ascontiguousarray(z, np.float64).copy()
+            assert Z.shape == (maxR + 1, n)
+        hist = np.zeros(hist_cap) if hist_cap else None
+        rs = C.c_uint32(restart_in)
+        self.L.orc_krylov_traced(kind, C.byref(M.c), C.byref(pcs), C.byref(params), _p(b),
+                                 _p(x), C.byref(res), _p(hist), hist_cap, C.byref(cyc), _p(V),
+                                 _p(Z), C.byref(rs))
+        k = min(cyc.count, cycle_cap)
+        out = {"retcode": res.retcode, "numIter": res.numIter, "norm2": res.norm2,
+               "normInf": res.normInf, "x": x,
+               "cycles": list(zip(ln[:k].tolist(), est[:k].tolist(), rtrue[:k].tolist())),
+               "planned": planned[:k].tolist(),
+               "V": V, "Z": Z, "dinv": dinv, "restart": rs.value}
+        if hist is not None:
+            out["hist"] = hist
+        return out
+
     # setup-time sparse algebra: each returns (nrow, mcol, rp, ci, va | None, dp)
     def _mat(self, fn, *args):
         o = _MatOut()
@@ -287,6 +343,12 @@ class Oracle:
 
     def mat_mult(self, L: Matrix, R: Matrix):
         return self._mat(self.L.orc_mat_mult, C.byref(L.c), C.byref(R.c))
+
+    def mat_shift(self, M: Matrix, a):
+        """MAT::Shift: the shifted values (M's dp must be set)."""
+        va = M.va.copy()
+        self.L.orc_mat_shift(C.byref(M.c), a, _p(va))
+        return va
 
     def mg(self, M: Matrix, b, x0, levels, ncycles=1, weight=1.0, hist_cap=0, **prm):
         """MG<MAT>::Solve with `levels` = [(transfer, smooth_prm, coarse_prm)]:
@@ -347,6 +409,8 @@ class Reference:
             "ref_mat_add": (C.c_int, [d, u, u, u, v, v, v, d, u, v, v, v,
                                       C.POINTER(_RefMatOut)]),
             "ref_mat_mult": (None, [u, u, u, v, v, v, u, u, v, v, v, C.POINTER(_RefMatOut)]),
+            "ref_mat_shift": (None, [u, u, u, v, v, v, d, v]),
+            "ref_omp_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -354,8 +418,12 @@ class Reference:
         self.L = L
 
     @staticmethod
-    def available() -> bool:
-        return REF_SO.exists()
+    def available(path=REF_SO) -> bool:
+        return pathlib.Path(path).exists()
+
+    def threads(self) -> int:
+        """Threads of the reference's OpenMP loops (1 for the serial build)."""
+        return self.L.ref_omp_threads()
 
     def _mat(self, fn, *args):
         o = _RefMatOut()
@@ -378,6 +446,12 @@ class Reference:
     def mat_mult(self, L: Matrix, R: Matrix):
         return self._mat(self.L.ref_mat_mult, L.nrow, L.mcol, L.nnz, _p(L.rp), _p(L.ci),
                          _p(L.va), R.mcol, R.nnz, _p(R.rp), _p(R.ci), _p(R.va))
+
+    def mat_shift(self, M: Matrix, a):
+        """MAT::Shift with the reference's own FormDiagPtr; the shifted values."""
+        out = np.empty(M.nnz)
+        self.L.ref_mat_shift(M.nrow, M.mcol, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), a, _p(out))
+        return out
 
     def _m(self, M: Matrix):
         return (M.nrow, M.mcol, M.nnz, _p(M.rp), _p(M.ci), _p(M.va), _p(M.dp))

@@ -16,6 +16,10 @@
 #include <cstring>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include "BiCGStab.hxx"
 #include "CG.hxx"
 #include "Iter.hxx"
@@ -48,6 +52,15 @@ void copy_out(const VEC& v, double* out) {
 }  // namespace
 
 extern "C" {
+
+// Threads the reference's OpenMP loops use (1 in the default serial build).
+int ref_omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 struct ref_params {
   uint32_t maxIter, minIter, savIter, restart;
@@ -292,6 +305,17 @@ void ref_mat_mult(uint32_t lrow, uint32_t lcol, uint32_t lnnz, const uint32_t* l
   MAT C;
   C.Mult(L, R);
   mat_out(C, out);
+}
+
+// MAT::Shift on a matrix whose diagPtr the reference forms itself
+// (FormDiagPtr in the constructor); the shifted values go to va_out.
+void ref_mat_shift(uint32_t nrow, uint32_t mcol, uint32_t nnz, const uint32_t* rp,
+                   const uint32_t* ci, const double* va, double a, double* va_out) {
+  MAT A = make_mat(nrow, mcol, nnz, rp, ci, va, nullptr);
+  A.Shift(a);
+  double* v = A.GetValues();
+  std::memcpy(va_out, v, sizeof(double) * nnz);
+  delete[] v;
 }
 
 }  // extern "C"

@@ -363,6 +363,58 @@ static void test_solvers() {
   }
 }
 
+// One CG object and one Jacobi preconditioner kept across a rebuilt MAT:
+// the matrix is assigned anew (another size, then the same size with other
+// values), Jacobi::Setup and CG::Setup run again, and each solve must still
+// reach the tolerance on the new system.
+static MAT poisson2d(USI m, double scale) {
+  std::vector<USI> rp(1, 0), ci;
+  std::vector<DBL> va;
+  const USI n = m * m;
+  for (USI i = 0; i < n; ++i) {
+    const USI ix = i % m, iy = i / m;
+    if (iy > 0) { ci.push_back(i - m); va.push_back(-scale); }
+    if (ix > 0) { ci.push_back(i - 1); va.push_back(-scale); }
+    ci.push_back(i); va.push_back(4 * scale + 0.01 * (i % 7));
+    if (ix < m - 1) { ci.push_back(i + 1); va.push_back(-scale); }
+    if (iy < m - 1) { ci.push_back(i + m); va.push_back(-scale); }
+    rp.push_back((USI)ci.size());
+  }
+  return MAT(n, n, (USI)ci.size(), va, ci, rp);
+}
+
+static void test_rebuild() {
+  MAT mat = poisson2d(64, 1.0);
+  class Jacobi pcd;
+  pcd.SetOutput(PRINT_NONE);
+  pcd.SetMaxIter(1);
+  pcd.SetMinIter(1);
+  class CG solver;
+  solver.SetMaxIter(2000);
+  solver.SetRelTol(1e-10);
+  solver.SetAbsTol(0.0);
+  solver.SetupPCD(pcd);
+  const struct { USI m; double scale; } steps[] = {{64, 1.0}, {48, 1.0}, {48, 2.5}, {64, 1.0}};
+  for (const auto& st : steps) {
+    mat = poisson2d(st.m, st.scale);
+    pcd.Setup(mat);
+    solver.Setup(mat);
+    const USI n = st.m * st.m;
+    VEC xs(n), b(n), x(n, 0.0), r(n);
+    for (USI i = 0; i < n; ++i) xs[i] = std::cos(0.02 * i);
+    mat.Apply(xs, b);
+    REQUIRE(solver.Solve(b, x) == SUCCESS);
+    mat.Apply(x, r);
+    r.XPAY(-1.0, b);
+    REQUIRE(r.Norm2() / b.Norm2() < 1e-9);
+    x.AXPY(-1.0, xs);
+    REQUIRE(x.Norm2() / xs.Norm2() < 1e-7);
+    std::printf("rebuild %ux%u scale %.1f: %u iterations\n", st.m, st.m, st.scale,
+                solver.GetIterations());
+  }
+  std::printf("ok rebuild\n");
+}
+
 // ReadMat (src/ReadData.cxx:361-446) into a device MAT, then CG on it:
 // nos7 as stored is its lower triangle, so solve with the normalised
 // fdm_10X10 system, which is SPD.  The data directory is passed in argv[1].
@@ -410,6 +462,7 @@ int main(int argc, char** argv) {
   test_matfree();
   test_jacobi();
   test_solvers();
+  test_rebuild();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
   return g_fail;
 }

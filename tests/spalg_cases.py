@@ -59,3 +59,31 @@ def aggregation(oracle, n, block, seed):
 def fixture(oracle, name):
     n, rp, ci, va = read_csr(name)
     return Matrix(n, n, rp, ci, va, oracle.form_diagptr(n, rp, ci))
+
+
+def missing_diag(oracle, n, m, density, seed, row0_diag=True):
+    """A random n x m matrix (entries U(-1, 1), columns sorted) in which about
+    a third of the rows 0..min(n, m)-1 carry their diagonal entry and the
+    rest do not (nor do rows >= m); diagPtr formed as FormDiagPtr does, so
+    the rows without a diagonal hold 0.  row0_diag: whether row 0 has its
+    diagonal (then values[0] is that entry, which the other rows' shifts also
+    land on)."""
+    rng = np.random.default_rng(seed)
+    S = sp.random(n, m, density=density, random_state=rng, format="lil")
+    k = min(n, m)
+    for i in range(k):
+        S[i, i] = 0.0
+    keep = rng.random(k) < 1 / 3
+    keep[0] = row0_diag
+    for i in np.flatnonzero(keep):
+        S[i, i] = 2.0 + rng.random()
+    S = S.tocsr()
+    S.eliminate_zeros()
+    S.sort_indices()
+    S.data = np.where(S.data == 0, 0.5, S.data)
+    rp = S.indptr.astype(np.uint32)
+    ci = S.indices.astype(np.uint32)
+    va = S.data.astype(np.float64)
+    if row0_diag:
+        assert ci[0] == 0
+    return Matrix(n, m, rp, ci, va, oracle.form_diagptr(n, rp, ci))

@@ -1,6 +1,6 @@
 """Device setup-time sparse algebra (fx_csr_lower_tri / upper_tri / add /
-mult, faspxx_amd/csrc/fx_spalg.hip) against the oracle's restatement of
-MAT::GetLowerTri, GetUpperTri, Add and Mult (itself pinned bit for bit
+mult / shift_diag, faspxx_amd/csrc/fx_spalg.hip) against the oracle's
+restatement of MAT::GetLowerTri, GetUpperTri, Add, Mult and Shift (itself pinned bit for bit
 against the compiled reference by tests/test_oracle_spalg.py).  Bar: the
 structure (rowPtr, colInd, diagPtr) and the values bit for bit; at full size
 (3D Poisson 128^3 squared) against SciPy's product, pattern exactly and
@@ -11,7 +11,8 @@ import scipy.sparse as sp
 
 from faspxx_amd import fx
 from oracle import Matrix
-from spalg_cases import aggregation, fixture, from_scipy, random_with_diag, to_scipy
+from spalg_cases import (aggregation, fixture, from_scipy, missing_diag, random_with_diag,
+                         to_scipy)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,3 +113,21 @@ def test_poisson3d_128_squared(ctx, problems):
     assert G.nnz == W.nnz
     assert np.array_equal(G.indptr, W.indptr) and np.array_equal(G.indices, W.indices)
     assert np.max(np.abs(G.data - W.data)) <= 1e-13 * np.max(np.abs(W.data))
+
+
+def test_shift(ctx, oracle, problems):
+    """MAT::Shift on matrices with and without their diagonal entries
+    (rows without one add to values[0]), square and rectangular, against
+    the oracle bit for bit; twice in a row on the same matrix."""
+    cases = [Matrix.of(problems(2, 16).A), random_with_diag(oracle, 2000, 0.005, 1),
+             missing_diag(oracle, 3000, 3000, 0.002, 11),
+             missing_diag(oracle, 2500, 2500, 0.003, 12, False),
+             missing_diag(oracle, 3000, 1800, 0.003, 13), missing_diag(oracle, 1500, 2600, 0.003, 14)]
+    for M in cases:
+        D = dev(ctx, M)
+        D.shift_diag(0.1)
+        want = oracle.mat_shift(M, 0.1)
+        assert np.array_equal(D.download()[2], want)
+        D.shift_diag(-3.75)
+        M2 = Matrix(M.nrow, M.mcol, M.rp, M.ci, want, M.dp)
+        assert np.array_equal(D.download()[2], oracle.mat_shift(M2, -3.75))

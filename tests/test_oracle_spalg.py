@@ -1,6 +1,6 @@
 """The oracle's setup-time sparse algebra (orc_mat_tri / orc_mat_add /
-orc_mat_mult: MAT::GetLowerTri, GetUpperTri, Add, Mult, src/MAT.cxx:290-345,
-652-800) pinned on the CPU: bit for bit against the compiled reference
+orc_mat_mult / orc_mat_shift: MAT::GetLowerTri, GetUpperTri, Add, Mult,
+Shift, src/MAT.cxx:290-365, 652-800) pinned on the CPU: bit for bit against the compiled reference
 (oracle/_ref, where it builds) on stencil, fixture, random and rectangular
 operands; against SciPy's sparse algebra (patterns exactly, values to
 rounding); and the product's column order (first appearance) against a
@@ -10,7 +10,8 @@ import pytest
 import scipy.sparse as sp
 
 from oracle import Matrix
-from spalg_cases import aggregation, fixture, from_scipy, random_with_diag, to_scipy
+from spalg_cases import (aggregation, fixture, from_scipy, missing_diag, random_with_diag,
+                         to_scipy)
 
 
 def same(a, b):
@@ -117,3 +118,41 @@ def test_mult_first_appearance_order(oracle):
     Ls = Matrix(L.nrow, L.mcol, L.rp, L.ci, None, L.dp)
     s = oracle.mat_mult(Ls, R)
     assert s[4] is None and np.array_equal(s[3], ci)
+
+
+def shift_cases(oracle, problems):
+    return [Matrix.of(problems(1, 16).A), random_with_diag(oracle, 300, 0.02, 1),
+            missing_diag(oracle, 200, 200, 0.03, 11), missing_diag(oracle, 150, 150, 0.04, 12, False),
+            missing_diag(oracle, 180, 120, 0.05, 13), missing_diag(oracle, 90, 140, 0.05, 14)]
+
+
+def test_shift_against_compiled_reference(oracle, reference, problems):
+    """Bit for bit, including rows without a diagonal entry (each adds to
+    values[0]) and rectangular matrices."""
+    for M in shift_cases(oracle, problems):
+        for a in (0.1, -3.75, 1e-3):
+            assert np.array_equal(oracle.mat_shift(M, a), reference.mat_shift(M, a))
+
+
+def test_shift_closed_form(oracle, problems):
+    """A full diagonal: exactly the diagonal entries change, each by a.
+    Missing diagonals: values[0] takes one sequential add of a per row whose
+    diagPtr is 0, and nothing else off the diagonal changes."""
+    M = random_with_diag(oracle, 300, 0.02, 1)
+    got = oracle.mat_shift(M, 0.25)
+    want = M.va.copy()
+    want[M.dp] += 0.25
+    assert np.array_equal(got, want)
+    for M in shift_cases(oracle, problems)[2:]:
+        a = 0.1
+        got = oracle.mat_shift(M, a)
+        rows = np.repeat(np.arange(M.nrow), np.diff(M.rp.astype(np.int64)))
+        isdiag = M.ci.astype(np.int64) == rows
+        hits0 = int(np.sum(M.dp == 0))
+        v0 = M.va[0]
+        for _ in range(hits0):
+            v0 = v0 + a
+        assert got[0] == v0
+        want = M.va.copy()
+        want[isdiag] += a
+        assert np.array_equal(got[1:], want[1:])
