@@ -231,6 +231,7 @@ static int ctx_init(fx_ctx c) {
   hipDeviceProp_t prop;
   FX_HIP(hipGetDeviceProperties(&prop, c->device));
   c->sms = prop.multiProcessorCount;
+  FX_TRY(noalloc_init(c->device, &c->scratch));  // the device is current here
   return 0;
 }
 

@@ -36,6 +36,10 @@ void clear_error();
 // and its device arrays, sizes and captured kernel arguments are then new.
 uint64_t next_handle_gen();
 
+// The device address of fx_pc_apply_noalloc's scratch array on `device`
+// (which must be current), taken once per device (fx_noalloc.hip).
+int noalloc_init(int device, double** out);
+
 }  // namespace fx
 
 #define FX_HIP(call)                                                                 \
@@ -81,6 +85,7 @@ struct fx_ctx_s {
   double* hpin = nullptr;   // pinned host mirror of dscal
   hipEvent_t tev[2] = {nullptr, nullptr};  // dispatch-attached timing events (fx_spmv_timed)
   int sms = 256;
+  double* scratch = nullptr;  // this device's fx_pc_apply_noalloc scratch (fx_noalloc.hip)
 };
 
 struct fx_vec_s {

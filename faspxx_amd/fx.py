@@ -142,6 +142,7 @@ _SIGS = {
     "fx_pc_create_jacobi": (C.c_int, [_P, _P, C.c_double, C.c_uint32, C.POINTER(_P)]),
     "fx_pc_destroy": (C.c_int, [_P]),
     "fx_pc_apply": (C.c_int, [_P, _P, _P, _P]),
+    "fx_pc_apply_noalloc": (C.c_int, [_P, _P, _P, _P]),
     "fx_solver_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "fx_solver_setup": (C.c_int, [_P, _P, _P, C.POINTER(Params)]),
     "fx_solver_solve": (C.c_int, [_P, _P, _P, C.POINTER(Result)]),
@@ -532,6 +533,11 @@ class Precond:
 
     def Solve(self, b: Vec, x: Vec):
         _chk(load().fx_pc_apply(self.ctx.h, self.h, b.h, x.h))
+
+    def SolveNoAlloc(self, b: Vec, x: Vec):
+        """Solve without device allocation (fx_pc_apply_noalloc): f64,
+        identity or Jacobi, at most 4,194,304 rows."""
+        _chk(load().fx_pc_apply_noalloc(self.ctx.h, self.h, b.h, x.h))
 
     def close(self):
         if self.h:

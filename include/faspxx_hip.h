@@ -256,6 +256,15 @@ int fx_pc_create_jacobi(fx_ctx ctx, fx_csr A, double weight, uint32_t iters, fx_
 int fx_pc_destroy(fx_pc pc);
 /* x <- P(b) with x as the initial guess: Jacobi::Solve, src/Iter.cxx:56-112 */
 int fx_pc_apply(fx_ctx ctx, fx_pc pc, fx_vec b, fx_vec x);
+/* No counterpart in the reference: fx_pc_apply for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same result bit for bit,
+ * with no allocation: identity and Jacobi preconditioners, f64 vectors, at
+ * most 4,194,304 rows (FX_ERROR_INPUT_PAR beyond).  Its work vector and
+ * reduction scratch live in one library-owned device array per GPU (32 MiB,
+ * held until the process ends) that every call on that GPU shares: calls
+ * that overlap on one GPU race. */
+int fx_pc_apply_noalloc(fx_ctx ctx, fx_pc pc, fx_vec b, fx_vec x);
 
 /* ---- Multicolor Gauss-Seidel smoother --------------------------------------
  * No reference counterpart: FASP++ has no Gauss-Seidel, and a sequential one
