@@ -485,7 +485,7 @@ struct BiRun {
   int launch_iteration_t(int c, BiState* hout) {
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid(A->nblk, cus);
+    const unsigned gs = spmv_grid_of<VT>(A, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
     BiSide* side = (BiSide*)W.side;
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{vp()}, BNone{},

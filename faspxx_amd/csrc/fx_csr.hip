@@ -329,11 +329,124 @@ bool patterns_disabled() {
   return e && e[0] == '1';
 }
 
+// Sliced rows: 0 off (FASPXX_FX_NOSELL=1), 1 for matrices whose columns
+// take neither row patterns nor 16-bit deltas (default), 2 for every matrix
+// made from host arrays (FASPXX_FX_SELL=1).
+int sell_mode() {
+  const char* e = std::getenv("FASPXX_FX_NOSELL");
+  if (e && e[0] == '1') return 0;
+  e = std::getenv("FASPXX_FX_SELL");
+  return (e && e[0] == '1') ? 2 : 1;
+}
+
+}  // namespace
+
+// The values of the slices from the CSR values: dst[p] = src[smap[p]]
+// (0 in padding slots).
+template <class T>
+__global__ void k_sell_fill(const uint32_t* __restrict__ smap, const T* __restrict__ src,
+                            T* __restrict__ dst, uint32_t slots) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < slots; p += stride) {
+    const uint32_t m = smap[p];
+    dst[p] = m == kSellNoRow ? (T)0 : src[m];
+  }
+}
+
+int sell_fill64(fx_csr A, const double* va, double* dst) {
+  if (!A->sslots) return 0;
+  hipLaunchKernelGGL(k_sell_fill<double>, dim3(stream_blocks(A->sslots)), dim3(kThreads), 0,
+                     A->ctx->stream, A->smap, va, dst, A->sslots);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+// Build the sliced form (for the f64 kernels) from the host row pointers
+// and columns.  The values are filled on the device from A->va, which must
+// be uploaded.
+int build_sell(fx_csr A, const uint32_t* hrp, const uint32_t* hci) {
+  const uint32_t n = A->nrow;
+  const uint32_t nwin = (n + kSellRows - 1) / kSellRows;
+  std::vector<uint32_t> swin(nwin + 1), lwin(nwin + 1), soff(1, 0), slane, lrow, scol, smap;
+  std::vector<uint32_t> sh, lo;
+  for (uint32_t w = 0; w < nwin; ++w) {
+    const uint32_t r0 = w * kSellRows, r1 = std::min(n, r0 + kSellRows);
+    swin[w] = (uint32_t)(soff.size() - 1);
+    lwin[w] = (uint32_t)lrow.size();
+    sh.clear();
+    lo.clear();
+    for (uint32_t r = r0; r < r1; ++r) (hrp[r + 1] - hrp[r] <= kSellMax ? sh : lo).push_back(r);
+    auto longer = [&](uint32_t a, uint32_t b) { return hrp[a + 1] - hrp[a] > hrp[b + 1] - hrp[b]; };
+    std::stable_sort(sh.begin(), sh.end(), longer);
+    std::stable_sort(lo.begin(), lo.end(), longer);
+    lrow.insert(lrow.end(), lo.begin(), lo.end());
+    for (size_t i = 0; i < sh.size(); i += 64) {
+      const uint32_t width = hrp[sh[i] + 1] - hrp[sh[i]];  // the slice's longest row
+      const uint32_t base = soff.back();
+      soff.push_back(base + width * 64);
+      scol.resize((size_t)base + width * 64, 0u);
+      smap.resize((size_t)base + width * 64, kSellNoRow);
+      for (uint32_t lane = 0; lane < 64; ++lane) {
+        if (i + lane >= sh.size()) {
+          slane.push_back(kSellNoRow);
+          continue;
+        }
+        const uint32_t r = sh[i + lane], len = hrp[r + 1] - hrp[r];
+        slane.push_back(((r - r0) << 16) | len);
+        for (uint32_t j = 0; j < len; ++j) {
+          const size_t p = (size_t)base + (size_t)j * 64 + lane;
+          scol[p] = hci[hrp[r] + j];
+          smap[p] = hrp[r] + j;
+        }
+      }
+    }
+  }
+  swin[nwin] = (uint32_t)(soff.size() - 1);
+  lwin[nwin] = (uint32_t)lrow.size();
+  A->nslice = (uint32_t)(soff.size() - 1);
+  A->sslots = soff.back();
+  hipStream_t st = A->ctx->stream;
+  auto up = [&](uint32_t** d, const std::vector<uint32_t>& h) -> int {
+    FX_HIP(hipMalloc(d, sizeof(uint32_t) * (h.empty() ? 1 : h.size())));
+    if (!h.empty())
+      FX_HIP(hipMemcpyAsync(*d, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice, st));
+    return 0;
+  };
+  FX_TRY(up(&A->swin, swin));
+  FX_TRY(up(&A->lwin, lwin));
+  FX_TRY(up(&A->soff, soff));
+  FX_TRY(up(&A->slane, slane));
+  FX_TRY(up(&A->lrow, lrow));
+  FX_TRY(up(&A->scol, scol));
+  FX_TRY(up(&A->smap, smap));
+  FX_HIP(hipMalloc(&A->sval, sizeof(double) * (A->sslots ? A->sslots : 1)));
+  if (A->has_values) FX_TRY(sell_fill64(A, A->va, A->sval));
+  // the windows' block table (first row, first entry), as blk
+  std::vector<uint32_t> blk(2 * ((size_t)nwin + 1));
+  for (uint32_t w = 0; w <= nwin; ++w) {
+    const uint32_t r = std::min(n, w * kSellRows);
+    blk[2 * w] = r;
+    blk[2 * w + 1] = hrp[r];
+  }
+  FX_TRY(up(&A->sblk, blk));
+  A->nwin = nwin;
+  A->sell = true;
+  FX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+  return 0;
+}
+
 int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
   // Row patterns for a matrix given on the host (FASPXX_FX_NOPAT=1: none;
   // FASPXX_FX_COL32=1 keeps u32 columns); they get their own row blocks.
+  // Sliced rows where neither patterns nor 16-bit deltas apply, or for
+  // every matrix given on the host with FASPXX_FX_SELL=1.
+  // FASPXX_FX_COL32=1 keeps the plain u32-column blocks (no slices).
+  const int sell = host_ci && A->nnz ? sell_mode() : 0;
   const char* e = std::getenv("FASPXX_FX_COL32");
-  const bool col32 = e && e[0] == '1';
+  const bool col32_env = e && e[0] == '1';
+  const bool col32 = col32_env || sell == 2;
   std::vector<uint8_t> pid;
   std::vector<int32_t> dict;
   uint32_t plen = 0, npat = 0;
@@ -397,6 +510,7 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
       FX_HIP(err);
     }
   }
+  if (!A->cd && (sell == 2 || (sell == 1 && !col32_env))) FX_TRY(build_sell(A, host_rp, host_ci));
   FX_HIP(hipStreamSynchronize(st));
   return 0;
 }
@@ -415,6 +529,9 @@ void csr_free(fx_csr A) {
   (void)hipFree(A->pid);
   (void)hipFree(A->pdict);
   (void)hipFree(A->src);
+  void* sell_arrays[] = {A->sblk, A->swin, A->soff, A->slane, A->scol,
+                         A->smap, A->sval, A->lwin, A->lrow};
+  for (void* q : sell_arrays) (void)hipFree(q);
   delete A;
 }
 
@@ -587,7 +704,15 @@ int fx_csr_destroy(fx_csr A) {
 int fx_csr_index_bytes(fx_csr A, int* bytes) {
   clear_error();
   FX_CHECK(A && bytes, FX_ERROR_INPUT_PAR, "null argument");
-  *bytes = A->pid ? 0 : A->cd ? 2 : 4;
+  *bytes = A->pid ? 0 : A->cd ? 2 : 4;  // sliced rows keep u32 columns
+  return 0;
+}
+
+int fx_csr_layout(fx_csr A, int* layout) {
+  clear_error();
+  FX_CHECK(A && layout, FX_ERROR_INPUT_PAR, "null argument");
+  *layout = A->pid ? FX_LAYOUT_PATTERN : A->cd ? FX_LAYOUT_D16 : A->sell ? FX_LAYOUT_SLICED
+                                                                       : FX_LAYOUT_U32;
   return 0;
 }
 
@@ -635,9 +760,11 @@ int fx_csr_set_values_dev(fx_csr A, const double* dev_values) {
   clear_error();
   FX_CHECK(A && dev_values, FX_ERROR_INPUT_PAR, "null argument");
   if (A->nnz == 0) return 0;
-  FX_HIP(hipMemcpyAsync(A->va, dev_values, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice,
-                        A->ctx->stream));
+  if (dev_values != A->va)
+    FX_HIP(hipMemcpyAsync(A->va, dev_values, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice,
+                          A->ctx->stream));
   A->has_values = true;
+  if (A->sell) FX_TRY(sell_fill64(A, A->va, A->sval));
   if (A->va32) return fx_csr_make_f32(A);
   return 0;
 }

@@ -818,7 +818,7 @@ struct CgRun {
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
                                                EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
                                                nullptr, false)));
-    const unsigned g1 = spmv_grid(A->nblk, cus);
+    const unsigned g1 = spmv_grid_of<VT>(A, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
     hipLaunchKernelGGL((k_cg_update_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, x, r, p, q,
                        dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
@@ -1188,7 +1188,7 @@ struct IrInner {
     FX_TRY((launch_spmv<float, float, ROW_AX, 1, 0u>(As, st, XLoad<float>{r}, BNone{},
                                                      EpiIrW{w, r, dstate(c)}, W.partials,
                                                      nullptr, nullptr, false)));
-    const unsigned g1 = spmv_grid(As->nblk, W.ctx->sms);
+    const unsigned g1 = spmv_grid_of<float>(As, W.ctx->sms);
     const unsigned g = wide_blocks(W.n, Pack16<float>::W, W.ctx->sms, kPassPacks);
     hipLaunchKernelGGL(k_ir_update, dim3(g), dim3(kWideThreads), 0, st, r, (const float*)w,
                        (float*)W.p, (float*)W.x2, (float*)W.x, W.n, dstate(c), dstate(c + 1),
@@ -1280,7 +1280,7 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
                           ? gmres_state_bytes()
                                                       : sizeof(CgState);
     int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
-                      spmv_grid(S->A->nblk, S->ctx->sms));
+                      spmv_grid_of<double>(S->A, S->ctx->sms));
     if (rc) {
       delete S->fast;
       S->fast = nullptr;
@@ -1441,7 +1441,8 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
     if (rc) return fail(rc);
   }
   const size_t nb = n ? n : 1;
-  const size_t np = 8 * ((size_t)spmv_grid(A->nblk, ctx->sms) + kMaxRedBlocks);
+  const size_t np = 8 * ((size_t)std::max(spmv_grid_of<double>(A), spmv_grid_of<float>(A)) +
+                         kMaxRedBlocks);
   if (hipMalloc(&ir->r32, sizeof(float) * nb) || hipMalloc(&ir->d32, sizeof(float) * nb) ||
       hipMalloc(&ir->dsc, sizeof(double) * 4) ||
       hipMalloc(&ir->part, sizeof(double) * np)) {
@@ -1451,7 +1452,7 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
   if (inner_type == FX_SOLVER_CG) {
     ir->W = new FastWs();
     const size_t sb = sizeof(CgState) > sizeof(IrState) ? sizeof(CgState) : sizeof(IrState);
-    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sb, spmv_grid(A->nblk, ctx->sms));
+    rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sb, spmv_grid_of<float>(A, ctx->sms));
     if (rc) return fail(rc);
   }
   *out = ir;
@@ -1485,7 +1486,7 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
             EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc}, ir->part, nullptr,
             nullptr, false);
         if (!rc)
-          rc = launch_finish<2, 0u>(st, ir->part, spmv_grid(A->nblk, ctx->sms),
+          rc = launch_finish<2, 0u>(st, ir->part, spmv_grid_of<double>(A, ctx->sms),
                                     FinIrInit{ir->dsc, (IrState*)ir->W->state, ir->inner.relTol,
                                               ir->inner.absTol, ir->inner.maxIter});
       } else {

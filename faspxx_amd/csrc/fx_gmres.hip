@@ -453,7 +453,7 @@ struct GmRun {
   // R: residual, its norms, state to the host slot
   int launch_residual(const VT* b, const VT* x, GmState* hout) {
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid(A->nblk, cus);
+    const unsigned gs = spmv_grid_of<VT>(A, cus);
     VT* v0 = vec(0);
     VT* tmp = (VT*)G.tmp;
     if (!left) {
@@ -475,7 +475,7 @@ struct GmRun {
   int launch_cycle(uint32_t m, const VT* b, VT* x) {
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid(A->nblk, cus);
+    const unsigned gs = spmv_grid_of<VT>(A, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
     const GmLsq L = lsq();
     const GmPrm P = prm(m);

@@ -135,6 +135,29 @@ struct fx_csr_s {
   uint32_t pat_rows = 256;  // rows per row block of a row-pattern matrix (256 or 512)
   // set on a transpose: entry p of this matrix is entry src[p] of its source
   uint32_t* src = nullptr;
+  // Sliced rows, for the f64 kernels of a matrix made from host arrays
+  // whose columns are too scattered for 16-bit deltas or row patterns
+  // (irregular matrices; the f32 kernels keep the row blocks above, which
+  // measured faster for them).  Row blocks of kSellRows rows ("windows",
+  // sblk / nwin in the format of blk / nblk).  In each window the rows of
+  // at most kSellMax entries are sorted by length (longest first) into
+  // slices of 64 rows stored column-major: slot soff[s] + j * 64 + lane
+  // holds entry j of the slice's lane-th row, so a wave reads a slice with
+  // whole-line loads and no LDS staging.  The window's longer rows are
+  // listed in lrow (longest first) and read from the CSR arrays.
+  bool sell = false;
+  uint32_t nwin = 0;
+  uint32_t* sblk = nullptr;
+  uint32_t nslice = 0, sslots = 0;  // slices; padded slots (soff[nslice])
+  uint32_t* swin = nullptr;   // nwin + 1: first slice of each window
+  uint32_t* soff = nullptr;   // nslice + 1: first slot of each slice
+  uint32_t* slane = nullptr;  // nslice * 64: (row - window's first row) << 16 | length,
+                              // 0xFFFFFFFF for a lane without a row
+  uint32_t* scol = nullptr;   // sslots: columns (padding: 0)
+  uint32_t* smap = nullptr;   // sslots: the CSR entry in each slot (padding: UINT32_MAX)
+  double* sval = nullptr;     // sslots: f64 values (padding: 0)
+  uint32_t* lwin = nullptr;   // nwin + 1: first long row of each window in lrow
+  uint32_t* lrow = nullptr;   // rows of more than kSellMax entries
 };
 
 struct fx_pc_s {
@@ -174,10 +197,20 @@ constexpr uint32_t kPatRows = 256;
 constexpr uint32_t kPatNnz = 7 * kPatRows;
 constexpr uint32_t kPat2Rows = 512;
 constexpr uint32_t kPat2Nnz = 7 * kPat2Rows;
+// Sliced rows (irregular matrices): windows of kSellRows rows, four slices
+// of 64 rows at most; rows of at most kSellMax entries go into slices.
+constexpr uint32_t kSellRows = kThreads;
+#ifndef FX_SELL_MAX
+#define FX_SELL_MAX 32
+#endif
+constexpr uint32_t kSellMax = FX_SELL_MAX;
+constexpr uint32_t kSellNoRow = 0xFFFFFFFFu;
 
 int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int ensure_dinv32(fx_pc pc);
+// Slice values of a sliced-row matrix from CSR-ordered values (fx_csr.hip).
+int sell_fill64(fx_csr A, const double* va, double* dst);
 
 // Host-side launch geometry for streaming kernels over n elements.
 inline unsigned stream_blocks(uint32_t n, uint32_t per_thread = 4) {
@@ -193,6 +226,18 @@ inline unsigned stream_blocks(uint32_t n, uint32_t per_thread = 4) {
 inline unsigned spmv_grid(uint32_t nblk, int /*cus*/) {
   unsigned g = nblk ? nblk : 1;
   return (g + kXcds - 1) / kXcds * kXcds;
+}
+
+// Row blocks an SpMV of matrix-value type MT runs on (one workgroup and one
+// reduction partial each): the sliced-row windows for f64 when the matrix
+// has them, else the row blocks.
+template <class MT>
+inline uint32_t spmv_units(const fx_csr_s* A) {
+  return (sizeof(MT) == 8 && A->sell) ? A->nwin : A->nblk;
+}
+template <class MT>
+inline unsigned spmv_grid_of(const fx_csr_s* A, int cus = 0) {
+  return spmv_grid(spmv_units<MT>(A), cus);
 }
 
 // ---------------------------------------------------------------------------

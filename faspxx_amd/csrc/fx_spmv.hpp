@@ -47,28 +47,43 @@ struct SpmvIn {
   const int32_t* __restrict__ pdict;   // offsets col - row, plen per pattern
   uint32_t plen;
   uint32_t dlen;  // npat * plen
+  // sliced rows (see fx_csr_s): slices of each window, their slots, lanes,
+  // columns and values; the windows' long rows
+  const uint32_t* __restrict__ swin;
+  const uint32_t* __restrict__ soff;
+  const uint32_t* __restrict__ slane;
+  const uint32_t* __restrict__ scol;
+  const MT* __restrict__ sval;  // null: structure-only matrix
+  const uint32_t* __restrict__ lwin;
+  const uint32_t* __restrict__ lrow;
 };
 
 // Column formats of the staged entries: u32 columns, 16-bit deltas to the
 // block's first row, or row patterns (no per-entry column data) in blocks
 // of 256 rows (CF_PAT) or 512 rows (CF_PAT2).
-enum { CF_U32 = 0, CF_D16 = 1, CF_PAT = 2, CF_PAT2 = 3 };
+enum { CF_U32 = 0, CF_D16 = 1, CF_PAT = 2, CF_PAT2 = 3, CF_SELL = 4 };
 constexpr bool is_pat(int cf) { return cf == CF_PAT || cf == CF_PAT2; }
 
 template <class MT, class VT>
 inline SpmvIn<MT, VT> spmv_in(fx_csr A);
 
+// f64 kernels of a sliced-row matrix run on its windows; everything else
+// on the row blocks (the f32 kernels get no slice arrays).
 template <>
 inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
   return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va : nullptr,
-          A->blk,  A->nblk, A->nnz, A->pid,   A->pdict,
-          A->plen, A->npat * A->plen};
+          A->sell ? A->sblk : A->blk,  A->sell ? A->nwin : A->nblk, A->nnz, A->pid,   A->pdict,
+          A->plen, A->npat * A->plen,
+          A->swin, A->soff, A->slane, A->scol, A->has_values ? A->sval : nullptr,
+          A->lwin, A->lrow};
 }
 template <>
 inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
   return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va32 : nullptr,
           A->blk,  A->nblk, A->nnz, A->pid,   A->pdict,
-          A->plen, A->npat * A->plen};
+          A->plen, A->npat * A->plen,
+          nullptr, nullptr, nullptr, nullptr, nullptr,
+          nullptr, nullptr};
 }
 
 // Threads per row for a block of nr rows holding nz entries: one thread per
@@ -122,9 +137,15 @@ template <class MT, int CF>
 struct SpmvLds {
   using RpT = std::conditional_t<FX_RP16 != 0, uint16_t, uint32_t>;
   using CiT = std::conditional_t<CF == CF_D16, int16_t, uint32_t>;
-  static constexpr uint32_t kRows = CF == CF_PAT2 ? kPat2Rows : CF == CF_PAT ? kPatRows : kBlockRows;
-  static constexpr uint32_t kNnz = CF == CF_PAT2 ? kPat2Nnz : CF == CF_PAT ? kPatNnz : kBlockNnz;
-  static constexpr uint32_t kCi = is_pat(CF) ? 1 : kBlockNnz;
+  static constexpr uint32_t kRows = CF == CF_SELL   ? 1
+                                    : CF == CF_PAT2 ? kPat2Rows
+                                    : CF == CF_PAT  ? kPatRows
+                                                    : kBlockRows;
+  static constexpr uint32_t kNnz = CF == CF_SELL   ? 1
+                                   : CF == CF_PAT2 ? kPat2Nnz
+                                   : CF == CF_PAT  ? kPatNnz
+                                                   : kBlockNnz;
+  static constexpr uint32_t kCi = (is_pat(CF) || CF == CF_SELL) ? 1 : kBlockNnz;
   static constexpr uint32_t kDict = is_pat(CF) ? kPatDict : 1;
   RpT rp[kRows + 1];
   CiT ci[kCi];
@@ -379,6 +400,148 @@ __device__ __forceinline__ void pat_block(const SpmvIn<MT, VT>& in, SpmvLds<MT, 
   }
 }
 
+// Rows longer than this in a sliced-row window are summed by the whole
+// workgroup rather than by one wave.
+#ifndef FX_SELL_HUGE
+#define FX_SELL_HUGE 256
+#endif
+constexpr uint32_t kSellHuge = FX_SELL_HUGE;
+// Entries per lane per step of a slice (loads, then gathers, then fmas).
+#ifndef FX_SELL_CHUNK
+#define FX_SELL_CHUNK 8
+#endif
+constexpr int kSellChunk = FX_SELL_CHUNK;
+
+// A window of a sliced-row matrix (CF_SELL): wave k sums slice k of the
+// window (64 rows of at most kSellMax entries, sorted by length), each lane
+// its own row left to right in the reference's order, reading the slice's
+// columns and values with whole-line loads straight into registers (8
+// entries' loads, then their 8 gathers, then the fma chain; slots past the
+// lane's length are clamped to its last entry and their terms dropped).
+// Then the window's long rows (more than kSellMax entries, longest first)
+// go round the waves, a wave per row: 4 strides of 64 entries in flight,
+// summed per lane in stride order, then across the wave; rows of more than
+// kSellHuge entries get the whole workgroup (4 strides of 256).  The row sums go
+// to LDS, and the epilogue then runs in row order, thread i on the
+// window's row i, so that its loads and stores are whole lines.
+template <int MODE, int NRED, class MT, class VT, class XF, class BF, class Epi>
+__device__ __forceinline__ void sell_block(const SpmvIn<MT, VT>& in, uint32_t w, int tid,
+                                           const XF& xf, const BF& bf, const Epi& epi,
+                                           double (&acc)[NRED > 0 ? NRED : 1]) {
+  __shared__ double rsum[kSellRows];
+  const int wid = tid >> 6, lane = tid & 63;
+  const uint32_t r0 = w * kSellRows;
+  const uint32_t nr = min(kSellRows, in.blk[2 * w + 2] - r0);
+  const uint32_t s = in.swin[w] + (uint32_t)wid;
+  if (s < in.swin[w + 1]) {
+    const uint32_t meta = stream_load<true>(&in.slane[s * 64u + (uint32_t)lane]);
+    const uint32_t base = in.soff[s], width = (in.soff[s + 1] - base) >> 6;
+    const bool has = meta != kSellNoRow;
+    const uint32_t len = has ? (meta & 0xFFFFu) : 0u, lm1 = len ? len - 1 : 0u;
+    const uint32_t row = r0 + (meta >> 16);
+    double t = (MODE == ROW_AX || !has) ? 0.0 : (double)bf(row);
+    const uint32_t* __restrict__ sc = in.scol + base + (uint32_t)lane;
+    const MT* __restrict__ sv = in.sval ? in.sval + base + (uint32_t)lane : nullptr;
+    // software pipelined: chunk k + 1's matrix loads go out before chunk k's
+    // gathers are waited for
+    uint32_t cj[kSellChunk];
+    MT av[kSellChunk];
+    if (width) {
+#pragma unroll
+      for (int u = 0; u < kSellChunk; ++u) {
+        const uint32_t j = min((uint32_t)u, lm1);
+        cj[u] = stream_load<true>(&sc[j * 64u]);
+        av[u] = sv ? stream_load<true>(&sv[j * 64u]) : (MT)1;
+      }
+    }
+    for (uint32_t j0 = 0; j0 < width; j0 += kSellChunk) {
+      double xv[kSellChunk];
+#pragma unroll
+      for (int u = 0; u < kSellChunk; ++u) xv[u] = (double)xf(cj[u]);
+      const MT* ac = av;
+      MT anext[kSellChunk];
+      uint32_t cnext[kSellChunk];
+      if (j0 + kSellChunk < width) {
+#pragma unroll
+        for (int u = 0; u < kSellChunk; ++u) {
+          const uint32_t j = min(j0 + kSellChunk + (uint32_t)u, lm1);
+          cnext[u] = stream_load<true>(&sc[j * 64u]);
+          anext[u] = sv ? stream_load<true>(&sv[j * 64u]) : (MT)1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSellChunk; ++u) {
+        const uint32_t j = j0 + (uint32_t)u;
+        if (j < len) {
+          const double a = (double)ac[u];
+          if (MODE == ROW_AX)
+            t = (j == 0) ? a * xv[u] : fma(a, xv[u], t);
+          else
+            t = fma(-a, xv[u], t);
+        }
+      }
+      if (j0 + kSellChunk < width) {
+#pragma unroll
+        for (int u = 0; u < kSellChunk; ++u) {
+          cj[u] = cnext[u];
+          av[u] = anext[u];
+        }
+      }
+    }
+    if (has) rsum[row - r0] = t;
+  }
+  // the window's long rows, longest first: those of more than kSellHuge
+  // entries are left to the whole workgroup below
+  const uint32_t q0 = in.lwin[w], q1 = in.lwin[w + 1];
+  uint32_t qh = q0;
+  while (qh < q1 && in.rp[in.lrow[qh] + 1] - in.rp[in.lrow[qh]] > kSellHuge) ++qh;
+  for (uint32_t q = qh + (uint32_t)wid; q < q1; q += kWaves) {
+    const uint32_t r = in.lrow[q], k0 = in.rp[r], k1 = in.rp[r + 1];
+    double t = 0.0;
+    for (uint32_t kb = k0 + (uint32_t)lane; kb < k1; kb += 4 * 64) {
+      double a[4], xv[4];
+      uint32_t c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t k = min(kb + (uint32_t)j * 64u, k1 - 1);
+        c[j] = stream_load<true>(&in.ci[k]);
+        a[j] = in.va ? (double)stream_load<true>(&in.va[k]) : 1.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (kb + (uint32_t)j * 64u < k1) t = fma(a[j], xv[j], t);
+    }
+    t = group_sum(t, 64);
+    if (lane == 0) rsum[r - r0] = (MODE == ROW_AX) ? t : (double)bf(r) - t;
+  }
+  for (uint32_t q = q0; q < qh; ++q) {
+    const uint32_t r = in.lrow[q], k0 = in.rp[r], k1 = in.rp[r + 1];
+    double t = 0.0;
+    for (uint32_t kb = k0 + (uint32_t)tid; kb < k1; kb += 4 * kThreads) {
+      double a[4], xv[4];
+      uint32_t c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t k = min(kb + (uint32_t)j * kThreads, k1 - 1);
+        c[j] = stream_load<true>(&in.ci[k]);
+        a[j] = in.va ? (double)stream_load<true>(&in.va[k]) : 1.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xv[j] = (double)xf(c[j]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (kb + (uint32_t)j * kThreads < k1) t = fma(a[j], xv[j], t);
+    }
+    double tt[1] = {t};
+    block_reduce<1>(tt);
+    if (tid == 0) rsum[r - r0] = (MODE == ROW_AX) ? tt[0] : (double)bf(r) - tt[0];
+  }
+  __syncthreads();
+  if ((uint32_t)tid < nr) epi.row(r0 + (uint32_t)tid, rsum[tid], acc);
+}
+
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
 // (a plain load, or a value formed on the fly); B gives b_i for
 // ROW_B_MINUS_AX.  Epi must provide:
@@ -400,7 +563,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
   for (int k = 0; k < (NRED > 0 ? NRED : 1); ++k) acc[k] = 0.0;
   const unsigned bid = xcd_block();
   const int tid = threadIdx.x;
-  if (bid < in.nblk) {
+  if constexpr (CF == CF_SELL) {
+    if (bid < in.nblk) sell_block<MODE, NRED>(in, bid, tid, xf, bf, epi, acc);
+  } else if (bid < in.nblk) {
     const uint32_t r0 = in.blk[2 * bid], k0 = in.blk[2 * bid + 1];
     const uint32_t r1 = in.blk[2 * bid + 2], k1 = in.blk[2 * bid + 3];
     const uint32_t nr = r1 - r0, nz = k1 - k0;
@@ -603,13 +768,16 @@ inline void launch_spmv_k(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, doubl
 template <class MT, class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
 inline int launch_spmv(fx_csr A, hipStream_t st, XF xf, BF bf, Epi epi, double* partials,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finisher = true) {
-  const unsigned g = spmv_grid(A->nblk, A->ctx->sms);
+  const unsigned g = spmv_grid_of<MT>(A, A->ctx->sms);
   if (A->pid && sizeof(MT) == 4 && A->pat_rows == kPat2Rows)  // f32 copies only
     launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi,
                   sizeof(MT) == 4 ? CF_PAT2 : CF_PAT>(A, st, xf, bf, epi, partials, e0, e1, g);
   else if (A->pid)
     launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_PAT>(A, st, xf, bf, epi, partials,
                                                                     e0, e1, g);
+  else if (A->sell && sizeof(MT) == 8)
+    launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_SELL>(A, st, xf, bf, epi,
+                                                                     partials, e0, e1, g);
   else if (A->cd)
     launch_spmv_k<MT, VT, MODE, NRED, MAXMASK, XF, BF, Epi, CF_D16>(A, st, xf, bf, epi, partials,
                                                                     e0, e1, g);

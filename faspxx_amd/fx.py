@@ -112,6 +112,7 @@ _SIGS = {
                                 C.POINTER(_P)]),
     "fx_csr_destroy": (C.c_int, [_P]),
     "fx_csr_index_bytes": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "fx_csr_layout": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "fx_spmv_timed": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_float)]),
     "fx_csr_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                               C.POINTER(C.c_uint32)]),
@@ -173,6 +174,15 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     if not p.exists():
         raise FileNotFoundError(
             f"{p} not found: build it with `python -m faspxx_amd.build` (no CPU fallback exists)")
+    # PyTorch ships its own HIP runtime under the same soname as the system
+    # one (libamdhip64.so.7), and whichever is loaded first serves the whole
+    # process.  Loading torch first makes it the runtime the library binds
+    # to, so torch tensors and the library share one runtime whatever the
+    # order in which the caller initialises them.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, (res, args) in _SIGS.items():
         f = getattr(lib, name)
@@ -436,6 +446,17 @@ class Csr:
         b = C.c_int()
         _chk(load().fx_csr_index_bytes(self.h, C.byref(b)))
         return b.value
+
+    @property
+    def layout(self) -> str:
+        """How the SpMV reads the matrix: 'u32', 'd16', 'pattern' or 'sliced'."""
+        b = C.c_int()
+        _chk(load().fx_csr_layout(self.h, C.byref(b)))
+        return ("u32", "d16", "pattern", "sliced")[b.value]
+
+    @property
+    def sliced(self) -> bool:
+        return self.layout == "sliced"
 
     def Apply(self, v: Vec, w: Vec):
         _chk(load().fx_spmv(self.ctx.h, self.h, v.h, w.h))

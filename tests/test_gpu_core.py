@@ -139,11 +139,13 @@ def test_column_delta_format(ctx, problems, monkeypatch, cid, scale, nbytes):
     16-bit reach (at n = 16384 every column is within 32K of every row)."""
     pr = problems(cid, scale)
     monkeypatch.setenv("FASPXX_FX_NOPAT", "1")  # row patterns off: deltas where they fit
+    monkeypatch.setenv("FASPXX_FX_NOSELL", "1")  # no sliced rows where deltas do not fit
     A = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.setenv("FASPXX_FX_COL32", "1")
     A32 = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.delenv("FASPXX_FX_COL32")
     monkeypatch.delenv("FASPXX_FX_NOPAT")
+    monkeypatch.delenv("FASPXX_FX_NOSELL")
     assert A.index_bytes == nbytes and A32.index_bytes == 4
     x = dev(ctx, np.random.default_rng(7).uniform(-1, 1, pr.A.nrow))
     w, w32 = fx.Vec(ctx, pr.A.nrow), fx.Vec(ctx, pr.A.nrow)
@@ -368,3 +370,123 @@ def test_empty_and_invalid(ctx):
     with pytest.raises(fx.FxError) as e:
         B.Apply(fx.Vec(ctx, 3), fx.Vec(ctx, 2))
     assert e.value.code == -16
+
+
+def _sliced_pair(ctx, Acsr, monkeypatch):
+    """The matrix with sliced rows (forced) and with plain u32-column blocks."""
+    monkeypatch.setenv("FASPXX_FX_SELL", "1")
+    S = fx.Csr.from_problem(ctx, Acsr)
+    monkeypatch.delenv("FASPXX_FX_SELL")
+    monkeypatch.setenv("FASPXX_FX_COL32", "1")
+    U = fx.Csr.from_problem(ctx, Acsr)
+    monkeypatch.delenv("FASPXX_FX_COL32")
+    return S, U
+
+
+def _check_sliced(ctx, oracle, Acsr, S, U, seed):
+    """Apply (f64, f32), Residual and the x = 0 shortcut: rows of at most 32
+    entries (sliced, one lane each, left to right) equal the oracle bit for
+    bit; longer rows (a wave each) agree to 1e-12 relative."""
+    n, m = Acsr.nrow, Acsr.mcol
+    M = Matrix.of(Acsr)
+    lens = np.diff(Acsr.row_ptr.astype(np.int64))
+    short = lens <= 32
+    xh = np.random.default_rng(seed).uniform(-1, 1, m)
+    w = fx.Vec(ctx, n)
+    S.Apply(dev(ctx, xh), w)
+    ref = oracle.spmv(M, xh)
+    got = w.numpy()
+    assert np.array_equal(got[short], ref[short])
+    if not short.all():
+        assert rel_err(got[~short], ref[~short]) < 1e-12
+    if n == m:
+        r = fx.Vec(ctx, n)
+        S.Residual(dev(ctx, Acsr_b(Acsr, seed)), dev(ctx, xh), r)
+        rref = oracle.residual(M, Acsr_b(Acsr, seed), xh)
+        assert np.array_equal(r.numpy()[short], rref[short])
+        if not short.all():
+            assert rel_err(r.numpy()[~short], rref[~short]) < 1e-12
+        S.Residual(dev(ctx, Acsr_b(Acsr, seed)), dev(ctx, np.zeros(n)), r)
+        assert np.array_equal(r.numpy(), Acsr_b(Acsr, seed))
+    # f32: short rows equal the plain u32 blocks' thread-per-row sums
+    S.make_f32()
+    U.make_f32()
+    x32 = dev(ctx, xh.astype(np.float32), fx.F32)
+    a, b = fx.Vec(ctx, n, fx.F32), fx.Vec(ctx, n, fx.F32)
+    S.Apply(x32, a)
+    U.Apply(x32, b)
+    if lens.max() <= 32:  # (longer rows change the plain blocks' lane groups)
+        assert np.array_equal(a.numpy()[short], b.numpy()[short])
+    assert rel_err(a.numpy(), b.numpy()) < 1e-6
+
+
+def Acsr_b(Acsr, seed):
+    return np.random.default_rng(seed + 100).uniform(-1, 1, Acsr.nrow)
+
+
+@pytest.mark.parametrize("cid,scale", [(1, None), (2, 32), (3, 256), (5, 64), (4, 16384),
+                                       (4, 200000)])
+def test_sliced_format(ctx, oracle, problems, monkeypatch, cid, scale):
+    """Sliced rows (the default for irregular matrices such as the power-law
+    system; forced here for the stencils) against the oracle."""
+    pr = problems(cid, scale)
+    S, U = _sliced_pair(ctx, pr.A, monkeypatch)
+    assert S.index_bytes == 4 and S.sliced and not U.sliced
+    _check_sliced(ctx, oracle, pr.A, S, U, cid)
+    if cid == 4 and scale == 200000:  # the default form there is sliced
+        D = fx.Csr.from_problem(ctx, pr.A)
+        assert D.sliced
+
+
+def test_sliced_rectangular_empty_and_long_rows(ctx, oracle, monkeypatch):
+    """A rectangular matrix (n > m and n < m) with empty rows, rows just at
+    and past the 32-entry slice limit, rows filling a whole window, and
+    windows without any short row; with values and structure-only."""
+    rng = np.random.default_rng(5)
+    for n, m in ((3000, 2200), (1500, 4000)):
+        lens = rng.integers(0, 12, n)
+        lens[rng.random(n) < 0.1] = 0
+        lens[rng.integers(0, n, 40)] = 32
+        lens[rng.integers(0, n, 40)] = 33
+        lens[rng.integers(0, n, 12)] = rng.integers(100, min(m, 1500), 12)
+        lens[512:768] = 40  # a window of long rows only
+        lens = np.minimum(lens, m)
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+        ci = np.concatenate([np.sort(rng.choice(m, k, replace=False)) for k in lens]).astype(
+            np.uint32)
+        va = rng.uniform(-1, 1, int(rp[-1]))
+        for values in (va, None):
+            monkeypatch.setenv("FASPXX_FX_SELL", "1")
+            S = fx.Csr(ctx, n, m, rp, ci, values, None)
+            monkeypatch.delenv("FASPXX_FX_SELL")
+            assert S.sliced
+            xh = rng.uniform(-1, 1, m)
+            w = fx.Vec(ctx, n)
+            S.Apply(dev(ctx, xh), w)
+            M = Matrix(n, m, rp, ci, values, oracle.form_diagptr(n, rp, ci))
+            ref = oracle.spmv(M, xh)
+            short = lens <= 32
+            assert np.array_equal(w.numpy()[short], ref[short])
+            assert rel_err(w.numpy()[~short], ref[~short]) < 1e-12
+
+
+def test_sliced_values_refresh(ctx, oracle, problems, monkeypatch):
+    """New values (set on the device, a diagonal shift) reach the slices."""
+    pr = problems(4, 16384)
+    monkeypatch.setenv("FASPXX_FX_SELL", "1")
+    A = fx.Csr.from_problem(ctx, pr.A)
+    monkeypatch.delenv("FASPXX_FX_SELL")
+    assert A.sliced
+    M = Matrix.of(pr.A)
+    xh = np.random.default_rng(3).uniform(-1, 1, pr.A.nrow)
+    lens = np.diff(pr.A.row_ptr.astype(np.int64))
+    short = lens <= 32
+    nv = dev(ctx, pr.A.values * -1.5)
+    A.set_values_dev(nv.data_ptr)
+    A.shift_diag(0.25)
+    want_vals = oracle.mat_shift(Matrix(M.nrow, M.mcol, M.rp, M.ci, pr.A.values * -1.5, M.dp), 0.25)
+    w = fx.Vec(ctx, pr.A.nrow)
+    A.Apply(dev(ctx, xh), w)
+    ref = oracle.spmv(Matrix(M.nrow, M.mcol, M.rp, M.ci, want_vals, M.dp), xh)
+    assert np.array_equal(w.numpy()[short], ref[short])
+    assert rel_err(w.numpy(), ref) < 1e-12
