@@ -411,6 +411,8 @@ class Reference:
             "ref_mat_mult": (None, [u, u, u, v, v, v, u, u, v, v, v, C.POINTER(_RefMatOut)]),
             "ref_mat_shift": (None, [u, u, u, v, v, v, d, v]),
             "ref_omp_threads": (C.c_int, []),
+            "ref_read_mat": (C.c_int, [C.c_char_p, C.POINTER(_RefMatOut)]),
+            "ref_read_csr": (C.c_int, [C.c_char_p, C.POINTER(_RefMatOut)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -446,6 +448,25 @@ class Reference:
     def mat_mult(self, L: Matrix, R: Matrix):
         return self._mat(self.L.ref_mat_mult, L.nrow, L.mcol, L.nnz, _p(L.rp), _p(L.ci),
                          _p(L.va), R.mcol, R.nnz, _p(R.rp), _p(R.ci), _p(R.va))
+
+    def _read(self, fn, path):
+        o = _RefMatOut()
+        rc = fn(str(path).encode(), C.byref(o))
+        if rc != 0:
+            return rc
+        try:
+            return (o.nrow, o.mcol) + _take(o, False)[:3]
+        finally:
+            self.L.ref_mat_free(C.byref(o))
+
+    def read_mat(self, path):
+        """ReadMat: (nrow, mcol, rp, ci, va) of the normalised CSRx, or the
+        reference's (nonzero) error code."""
+        return self._read(self.L.ref_read_mat, path)
+
+    def read_csr(self, path):
+        """ReadCSR: the file's raw arrays, or the reference's error code."""
+        return self._read(self.L.ref_read_csr, path)
 
     def mat_shift(self, M: Matrix, a):
         """MAT::Shift with the reference's own FormDiagPtr; the shifted values."""

@@ -25,6 +25,7 @@
 #include "Iter.hxx"
 #include "MAT.hxx"
 #include "MATUtil.hxx"
+#include "ReadData.hxx"
 #include "VEC.hxx"
 
 namespace {
@@ -305,6 +306,39 @@ void ref_mat_mult(uint32_t lrow, uint32_t lcol, uint32_t lnnz, const uint32_t* l
   MAT C;
   C.Mult(L, R);
   mat_out(C, out);
+}
+
+// ReadMat (src/ReadData.cxx:361-446): a .csr or .mtx file read and
+// normalised to CSRx by the reference (SortCSRRow, CSRtoMAT / MTXtoMAT).
+// Returns the reference's FaspRetCode; out is filled on success.
+int ref_read_mat(const char* file, ref_mat_out* out) {
+  MAT A;
+  const FaspRetCode rc = ReadMat(file, A);
+  if (rc != FaspRetCode::SUCCESS) return (int)rc;
+  mat_out(A, out);
+  return 0;
+}
+
+// ReadCSR (src/ReadData.cxx:221-357): the raw arrays as the file holds them
+// (0-based), before any normalisation.
+int ref_read_csr(const char* file, ref_mat_out* out) {
+  USI row = 0, col = 0, nnz = 0;
+  std::vector<USI> rp, ci;
+  std::vector<DBL> va;
+  const FaspRetCode rc = ReadCSR(file, row, col, nnz, rp, ci, va);
+  if (rc != FaspRetCode::SUCCESS) return (int)rc;
+  out->nrow = row;
+  out->mcol = col;
+  out->nnz = nnz;
+  out->rp = (uint32_t*)std::malloc(sizeof(uint32_t) * (row + 1));
+  out->ci = (uint32_t*)std::malloc(sizeof(uint32_t) * (nnz ? nnz : 1));
+  out->va = (double*)std::malloc(sizeof(double) * (nnz ? nnz : 1));
+  for (USI i = 0; i <= row; ++i) out->rp[i] = rp[i];
+  for (USI k = 0; k < nnz; ++k) {
+    out->ci[k] = ci[k];
+    out->va[k] = va[k];
+  }
+  return 0;
 }
 
 // MAT::Shift on a matrix whose diagPtr the reference forms itself

@@ -1,10 +1,12 @@
 """File ingestion of the drop-in layer (faspxx_amd/host/faspxx_readdata.cpp:
 ReadCSR / ReadMTX / ReadMat with SortCSRRow and CSRtoMAT's zero-diagonal
 insertion) on the reference's data files, CPU only.  The expected CSRx arrays
-come from independent normalisations: tests/csrfile.py for .csr, and for .mtx
-a NumPy one written from the format (rows bucketed in file order, a stable
-sort by column, zero diagonals inserted).  Structure and values must match
-exactly; the oracle's CheckCSRx must accept the result."""
+come from the compiled reference's own ReadMat / ReadCSR (src/ReadData.cxx,
+oracle/_ref) where it is built, and from independent normalisations:
+tests/csrfile.py for .csr, and for .mtx a NumPy one written from the format
+(rows bucketed in file order, a stable sort by column, zero diagonals
+inserted).  Structure and values must match exactly; the oracle's CheckCSRx
+must accept the result."""
 import pathlib
 import subprocess
 
@@ -87,3 +89,30 @@ def test_raw_csr_is_the_file():
     nnz = int(frp[-1] - frp[0])
     assert np.array_equal(ci, np.array(tok[n + 2:n + 2 + nnz], dtype=np.int64) - shift)
     assert np.array_equal(va, np.array(tok[n + 2 + nnz:n + 2 + 2 * nnz], dtype=np.float64))
+
+
+@pytest.mark.parametrize("name", ["fdm_10X10.csr", "fem_small.csr", "nos7.mtx", "nos7-.mtx"])
+def test_readmat_against_compiled_reference(reference, name):
+    """The normalised CSRx equals the reference's ReadMat bit for bit."""
+    rc, row, col, rp, ci, va = run("csrx", name)
+    ref = reference.read_mat(GOLD / name)
+    assert not isinstance(ref, int), ref
+    assert rc == 0 and (row, col) == ref[:2]
+    assert np.array_equal(rp, ref[2]) and np.array_equal(ci, ref[3])
+    assert np.array_equal(va, ref[4])
+
+
+@pytest.mark.parametrize("name", ["fdm_10X10.csr", "fem_small.csr"])
+def test_readcsr_raw_against_compiled_reference(reference, name):
+    rc, row, col, rp, ci, va = run("raw", name)
+    ref = reference.read_csr(GOLD / name)
+    assert not isinstance(ref, int), ref
+    assert rc == 0 and (row, col) == ref[:2]
+    assert np.array_equal(rp, ref[2]) and np.array_equal(ci, ref[3])
+    assert np.array_equal(va, ref[4])
+
+
+def test_missing_file_code_matches_reference(reference):
+    rc = subprocess.run([str(BIN), "csrx", str(GOLD / "missing.csr")], capture_output=True,
+                        text=True).stdout.split()
+    assert int(rc[0]) == reference.read_mat(GOLD / "missing.csr") == -10  # ERROR_OPEN_FILE
