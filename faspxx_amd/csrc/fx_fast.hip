@@ -139,8 +139,11 @@ __device__ __forceinline__ void cg_after_update(CgState* S, const CgPrm& P, cons
 template <class VT, bool JAC>
 struct CgUpd {
   VT a;
+  // N = 6 (a multi-sweep Jacobi preconditioner): slot 5 is max |z| of the
+  // first sweep, for the next sweep's MAT::Residual shortcut
+  template <int N>
   __device__ __forceinline__ void elem(VT pi, VT qi, VT& xi, VT& ri, VT di,
-                                       double (&acc)[5]) const {
+                                       double (&acc)[N]) const {
     xi = fma(a, pi, xi);   // x.AXPY(alpha, p)
     ri = fma(-a, qi, ri);  // r.AXPY(-alpha, q)
     const VT zi = JAC ? ri * di : ri;  // Jacobi sweep from z = 0: z = r .* dinv
@@ -149,6 +152,7 @@ struct CgUpd {
     acc[2] = fmax(acc[2], fabs((double)xi));
     acc[3] = fma((double)xi, (double)xi, acc[3]);
     acc[4] = fma((double)pi, (double)pi, acc[4]);
+    if constexpr (N > 5) acc[5] = fmax(acc[5], fabs((double)zi));
   }
 };
 
@@ -196,12 +200,15 @@ struct UpdPass {
       else d[i] = Pack16<VT>{};
     }
   }
+  template <int N>
   __device__ __forceinline__ void run(const CgUpd<VT, JAC>& u, VT* __restrict__ xv,
-                                      VT* __restrict__ rv, double (&acc)[5]) {
+                                      VT* __restrict__ rv, double (&acc)[N]) {
     constexpr int W = Pack16<VT>::W;
 #pragma unroll
     for (int i = 0; i < kPassPacks; ++i) {
-      double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      double a[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) a[k] = 0.0;
 #pragma unroll
       for (int w = 0; w < W; ++w) u.elem(p[i].v[w], q[i].v[w], x[i].v[w], r[i].v[w], d[i].v[w], a);
       if (ok[i]) {
@@ -209,15 +216,15 @@ struct UpdPass {
         st16(rv, c[i], r[i]);
       }
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < N; ++k) {
         const double b = ok[i] ? a[k] : 0.0;
-        acc[k] = k == 2 ? fmax(acc[k], b) : acc[k] + b;
+        acc[k] = (k == 2 || k == 5) ? fmax(acc[k], b) : acc[k] + b;
       }
     }
   }
 };
 
-template <class VT, bool JAC>
+template <class VT, bool JAC, bool SW = false>
 __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
     VT* __restrict__ x, VT* __restrict__ r, const VT* __restrict__ p, const VT* __restrict__ q,
     const VT* __restrict__ dinv, uint32_t n, const CgState* __restrict__ st,
@@ -241,7 +248,10 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
   }
   if (brk) return;  // the same decision in every workgroup
   const CgUpd<VT, JAC> u{(VT)(tmpa / pq[0])};
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr int NA = SW ? 6 : 5;
+  double acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   pass.run(u, x, r, acc);  // stores only packs below nc
   for (uint32_t c = cfirst + kPassPacks * stride; c < nc; c += kPassPacks * stride) {
     pass.load(p, q, x, r, dinv, c, stride, nc);
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
     x[i] = xi;
     r[i] = ri;
   }
-  block_partials_n<5, 4u, kWideThreads>(acc, p2);
+  block_partials_n<NA, SW ? (4u | 32u) : 4u, kWideThreads>(acc, p2);
 }
 
 // Packs per thread per pass of K3: 2 measured 20.1 against 20.3 ms for the
@@ -263,7 +273,9 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_update_w(
 #define FX_PPACKS 2
 #endif
 constexpr int kPPacks = FX_PPACKS;
-template <class VT, bool JAC>
+// SW: z is a vector (the last of several Jacobi sweeps), loaded in place of
+// r, and no dinv.
+template <class VT, bool JAC, bool SW = false>
 struct PPass {
   Pack16<VT> p[kPPacks], r[kPPacks], d[kPPacks];
   uint32_t c[kPPacks];
@@ -278,7 +290,7 @@ struct PPass {
       c[i] = ok[i] ? ci : 0u;
       p[i] = ld16(pv, c[i]);
       r[i] = ld16(rv, c[i]);
-      if constexpr (JAC) d[i] = ld16(dinv, c[i]);
+      if constexpr (JAC && !SW) d[i] = ld16(dinv, c[i]);
       else d[i] = Pack16<VT>{};
     }
   }
@@ -288,17 +300,21 @@ struct PPass {
     for (int i = 0; i < kPPacks; ++i) {
 #pragma unroll
       for (int w = 0; w < W; ++w)
-        p[i].v[w] = fma(beta, p[i].v[w], JAC ? r[i].v[w] * d[i].v[w] : r[i].v[w]);  // p.XPAY(beta, z)
+        p[i].v[w] = fma(beta, p[i].v[w],
+                        (JAC && !SW) ? r[i].v[w] * d[i].v[w] : r[i].v[w]);  // p.XPAY(beta, z)
       if (ok[i]) st16(pv, c[i], p[i]);
     }
   }
 };
 
-template <class VT, bool JAC>
+// SW (several Jacobi sweeps): r is the last sweep's z, and (z, r) comes
+// from that sweep's partials pz (gz of them) instead of K2's slot 1.
+template <class VT, bool JAC, bool SW = false>
 __global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
     VT* __restrict__ p, const VT* __restrict__ r, const VT* __restrict__ dinv, uint32_t n,
     CgState* st_cur, CgState* st_next, CgState* hout, const double* __restrict__ p2,
-    unsigned g2, const CgSide* __restrict__ side, CgPrm P, double* hist) {
+    unsigned g2, const CgSide* __restrict__ side, CgPrm P, double* hist,
+    const double* __restrict__ pz = nullptr, unsigned gz = 0) {
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (st_cur->status != ST_RUN) {  // stopped: hand the state on unchanged
     if (lead) {
@@ -310,10 +326,15 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
   constexpr int W = Pack16<VT>::W;
   const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
   const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
-  PPass<VT, JAC> pass;  // loaded before beta is known (nc == 0: pack 0 of the padded vectors)
+  PPass<VT, JAC, SW> pass;  // loaded before beta is known (nc == 0: pack 0 of the padded vectors)
   pass.load(p, r, dinv, cfirst, stride, nc);
   double a[5];
   partials_thread<5, 4u, kWideThreads>(p2, g2, threadIdx.x, a);
+  if constexpr (SW) {
+    double zr[1];
+    partials_thread<1, 0u, kWideThreads>(pz, gz, threadIdx.x, zr);
+    a[1] = zr[0];
+  }
   partials_combine<5, 4u, kWideThreads>(a);
   __shared__ double sh_beta;
   __shared__ int sh_run;
@@ -348,8 +369,61 @@ __global__ __launch_bounds__(kWideThreads) void k_cg_pupdate_w(
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
     const uint32_t i = nc * W + threadIdx.x;
-    p[i] = fma(beta, p[i], JAC ? r[i] * dinv[i] : r[i]);
+    p[i] = fma(beta, p[i], (JAC && !SW) ? r[i] * dinv[i] : r[i]);
   }
+}
+
+// ---- a Jacobi preconditioner of several sweeps inside the CG iteration ------
+// Jacobi::Solve with minIter == maxIter == k from z = 0 (src/Iter.cxx:56-112,
+// the preconditioner of example/TestPCG.cxx:71-76): K2 forms the first sweep
+// z1 = r .* dinv inline (with max |z1|); each further sweep is an SpMV in
+// Residual form, work = r - A z (or r when ||z||_inf < 1e-20,
+// src/MAT.cxx:459-462), z += work .* dinv, preceded by a one-workgroup
+// max-norm finisher; the last sweep also gives the partials of (z, r).
+template <class VT>
+struct XJacZ {  // z1 = r .* dinv, formed where the first extra sweep gathers it
+  const VT* __restrict__ r;
+  const VT* __restrict__ dinv;
+  __device__ __forceinline__ VT operator()(uint32_t j) const { return r[j] * dinv[j]; }
+};
+
+template <class VT, bool LAST>
+struct EpiCgSweep {
+  VT* __restrict__ zn;        // the new z
+  const VT* __restrict__ zc;  // the current z (null: z1 = r .* dinv)
+  const VT* __restrict__ r;
+  const VT* __restrict__ dinv;
+  const double* __restrict__ zinf;  // ||z||_inf of the current z
+  const CgState* st;
+  __device__ bool skip() const { return st->status != ST_RUN; }
+  __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[1]) const {
+    const VT w = (*zinf < kCloseZero) ? r[i] : (VT)v;  // MAT::Residual
+    const VT zcur = zc ? zc[i] : r[i] * dinv[i];
+    const VT z = zcur + w * dinv[i];  // work.PointwiseMult(diagInv); z += work
+    zn[i] = z;
+    if constexpr (LAST)
+      acc[0] = fma((double)z, (double)r[i], acc[0]);
+    else
+      acc[0] = fmax(acc[0], fabs((double)z));
+  }
+  __device__ void fin(double (&)[1]) const {}
+};
+
+// p = z, and the partials of (r, r), (z, r) for the loop start (z given).
+template <class VT>
+__global__ __launch_bounds__(kThreads) void k_cg_init_z(VT* __restrict__ p,
+                                                        const VT* __restrict__ r,
+                                                        const VT* __restrict__ z, uint32_t n,
+                                                        double* partials) {
+  double acc[2] = {0.0, 0.0};
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const VT ri = r[i], zi = z[i];
+    p[i] = zi;
+    acc[0] = fma((double)ri, (double)ri, acc[0]);
+    acc[1] = fma((double)zi, (double)ri, acc[1]);
+  }
+  block_partials<2>(acc, partials);
 }
 
 // z = P(r), p = z, tmpa = (z, r), and the first loop-top check.
@@ -801,6 +875,7 @@ struct CgRun {
   const VT* dinv;  // null: identity preconditioner
   CgPrm P;
   hipStream_t st;
+  uint32_t sweeps = 1;  // Jacobi preconditioner sweeps (z buffers W.s, W.t when > 1)
 
   // state buffer of parity c (iteration c of a replay reads c & 1)
   CgState* dstate(int c = 0) const { return (CgState*)W.state + (c & 1); }
@@ -830,7 +905,66 @@ struct CgRun {
     FX_HIP(hipGetLastError());
     return 0;
   }
+
+  // The iteration with a Jacobi preconditioner of `sweeps` > 1 sweeps: K1,
+  // K2 (with max |z1|), then per further sweep a max-norm finisher and the
+  // sweep's SpMV (z in W.s / W.t alternately; partials in part1, which K2
+  // has consumed), then K3 with the last z and its (z, r).
+  int launch_iteration_sweeps(int c, CgState* hout) {
+    VT* p = (VT*)W.p;
+    VT* q = (VT*)W.q;
+    VT* x = (VT*)W.x;
+    VT* r = (VT*)W.r;
+    const uint32_t n = W.n;
+    const int cus = W.ctx->sms;
+    FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
+                                               EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
+                                               nullptr, false)));
+    const unsigned g1 = spmv_grid_of<VT>(A, cus);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
+    hipLaunchKernelGGL((k_cg_update_w<VT, true, true>), dim3(g), dim3(kWideThreads), 0, st, x, r,
+                       p, q, dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
+                       (CgSide*)W.side, part2());
+    FX_HIP(hipGetLastError());
+    VT* zb[2] = {(VT*)W.s, (VT*)W.t};
+    const VT* zc = nullptr;  // z1 = r .* dinv, not stored
+    double* zinf = W.dscal;
+    for (uint32_t j = 1; j < sweeps; ++j) {
+      const bool last = j + 1 == sweeps;
+      // ||z_j||_inf: K2's slot 5 for z1, else the previous sweep's partials
+      if (j == 1)
+        FX_TRY((launch_finish<1, 1u>(st, part2() + 5 * (size_t)g, g, FinStore<1>{zinf, 0})));
+      else
+        FX_TRY((launch_finish<1, 1u>(st, part1(), g1, FinStore<1>{zinf, 0})));
+      VT* zn = zb[(j - 1) & 1];
+      const CgState* cs = dstate(c);
+      if (j == 1 && last)
+        FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 1, 0u>(
+            A, st, XJacZ<VT>{r, dinv}, BLoad<VT>{r}, EpiCgSweep<VT, true>{zn, zc, r, dinv, zinf, cs},
+            part1(), nullptr, nullptr, false)));
+      else if (j == 1)
+        FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 1, 1u>(
+            A, st, XJacZ<VT>{r, dinv}, BLoad<VT>{r},
+            EpiCgSweep<VT, false>{zn, zc, r, dinv, zinf, cs}, part1(), nullptr, nullptr, false)));
+      else if (last)
+        FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 1, 0u>(
+            A, st, XLoad<VT>{zc}, BLoad<VT>{r}, EpiCgSweep<VT, true>{zn, zc, r, dinv, zinf, cs},
+            part1(), nullptr, nullptr, false)));
+      else
+        FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 1, 1u>(
+            A, st, XLoad<VT>{zc}, BLoad<VT>{r}, EpiCgSweep<VT, false>{zn, zc, r, dinv, zinf, cs},
+            part1(), nullptr, nullptr, false)));
+      zc = zn;
+    }
+    hipLaunchKernelGGL((k_cg_pupdate_w<VT, true, true>), dim3(g), dim3(kWideThreads), 0, st, p,
+                       zc, dinv, n, dstate(c), dstate(c + 1), hout, (const double*)part2(), g,
+                       (const CgSide*)W.side, P, W.hist, (const double*)part1(), g1);
+    FX_HIP(hipGetLastError());
+    return 0;
+  }
+
   int launch_iteration(int c, CgState* hout) {
+    if (dinv && sweeps > 1) return launch_iteration_sweeps(c, hout);
     return dinv ? launch_iteration_t<true>(c, hout) : launch_iteration_t<false>(c, hout);
   }
   CgState* hslot(int slot) const { return (CgState*)W.hstate + slot; }
@@ -998,12 +1132,25 @@ struct CgRun {
     FX_TRY((launch_spmv<VT, VT, ROW_AX, 0, 0u>(A, st, XLoad<VT>{(const VT*)W.x}, BNone{},
                                                EpiBMinusW<VT>{(VT*)W.r, b}, nullptr)));
     const unsigned g = stream_blocks(n);
-    if (dinv)
+    if (dinv && sweeps > 1) {
+      // z = P r from z = 0 with every sweep (src/CG.cxx:74-75), into W.s
+      fx_vec vr0 = nullptr, vz0 = nullptr;
+      int rc = fx_vec_wrap(W.ctx, n, pc_dtype, W.r, &vr0);
+      if (!rc) rc = fx_vec_wrap(W.ctx, n, pc_dtype, W.s, &vz0);
+      if (!rc) rc = fx_vec_set(vz0, 0.0);
+      if (!rc) rc = fx_pc_apply(W.ctx, S->pc, vr0, vz0);
+      fx_vec_destroy(vr0);
+      fx_vec_destroy(vz0);
+      FX_TRY(rc);
+      hipLaunchKernelGGL((k_cg_init_z<VT>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
+                         (const VT*)W.r, (const VT*)W.s, n, W.partials);
+    } else if (dinv) {
       hipLaunchKernelGGL((k_cg_init<VT, true>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
                          (const VT*)W.r, dinv, n, W.partials);
-    else
+    } else {
       hipLaunchKernelGGL((k_cg_init<VT, false>), dim3(g), dim3(kThreads), 0, st, (VT*)W.p,
                          (const VT*)W.r, dinv, n, W.partials);
+    }
     FX_HIP(hipGetLastError());
     FX_TRY((launch_finish<2, 0u>(st, W.partials, g, FinCgInit{dstate(), P, W.hist})));
     FX_TRY(build_graph());
@@ -1260,11 +1407,11 @@ bool fast_eligible(fx_solver S) {
   if (fast_disabled() || S->A->nrow == 0) return false;
   const bool pc_ok =
       S->pc->kind == 0 || (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A == S->A);
-  if (S->type == FX_SOLVER_CG || S->type == FX_SOLVER_BICGSTAB) return pc_ok;
-  // GMRES / FGMRES: the safe-guard copy (savIter <= maxIter) stays on the
-  // per-operation path
-  if (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
-    return pc_ok && S->prm.savIter > S->prm.maxIter;
+  // CG also takes a Jacobi preconditioner of several sweeps
+  const bool pc_cg = pc_ok || (S->pc->kind == 1 && S->pc->iters > 1 && S->pc->A == S->A);
+  if (S->type == FX_SOLVER_CG) return pc_cg;
+  if (S->type == FX_SOLVER_BICGSTAB) return pc_ok;
+  if (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES) return pc_ok;
   if (S->type == FX_SOLVER_JACOBI) return S->pc->A == S->A;
   return false;
 }
@@ -1295,6 +1442,12 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
     return gmres_fast(S, W, (const double*)b->d, (double*)x->d, res);
   CgRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
                     S->ctx->stream};
+  if (S->pc->kind == 1 && S->pc->iters > 1) {
+    run.sweeps = S->pc->iters;
+    const size_t vb = pack_round(W.n) * sizeof(double);
+    if (!W.s) FX_HIP(hipMalloc(&W.s, vb));
+    if (!W.t) FX_HIP(hipMalloc(&W.t, vb));
+  }
   return run.run((const double*)b->d, (double*)x->d, res, S->prm, FX_F64);
 }
 

@@ -44,6 +44,9 @@ enum { GM_RUN = 0, GM_CYCLE_END = 1 };
 struct GmState {
   double resAbs, rj, denAbs, pad0;
   uint32_t numIter, count, status, histCount;
+  // the cycle's last residual estimate and the reference's resAbsOld after
+  // it (src/GMRES.cxx:198-201), for the safe-guard test after the x update
+  double est, est_old;
 };
 
 // Device-resident least-squares data of one cycle: H is (maxR+1) x maxR,
@@ -232,6 +235,8 @@ __global__ __launch_bounds__(kWideThreads) void k_gm_givens(
     S.count = cnt;
     givens_column(L, cnt);
     S.resAbs = fabs(L.g[cnt]);
+    S.est = S.resAbs;
+    S.est_old = S.resAbs;  // resAbsOld = resAbs after the ratio, :198-201
     const double resRel = S.resAbs / S.denAbs;
     const bool conv = (S.resAbs < P.absTol || resRel < P.relTol) && S.numIter > P.minIter;
     if (conv || cnt >= P.m || S.numIter >= P.maxIter) {
@@ -569,6 +574,12 @@ struct GmRun {
     solve_restart(S, maxR, minR, restart);
     VT* x = (VT*)W.x;
     FX_HIP(hipMemcpyAsync(x, x_user, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
+    // safe-guard (savIter <= maxIter): safe = x0, then x after a cycle whose
+    // estimate improved, and x = safe at the end past savIter
+    // (src/GMRES.cxx:115-122,234,277; src/FGMRES.cxx:123,224,268)
+    const bool guard = PP.savIter <= PP.maxIter;
+    VT* safe = (VT*)W.safe;
+    if (guard) FX_HIP(hipMemcpyAsync(safe, x, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
     if (flex) {  // the stale Z of earlier solves: their max-norms
       const unsigned gz = stream_blocks(n);
       hipLaunchKernelGGL((k_zinf_part<VT>), dim3(gz, maxR + 1), dim3(kThreads), 0, st,
@@ -603,6 +614,8 @@ struct GmRun {
       FX_TRY(cycle(restart, bw, x));
       FX_HIP(hipStreamSynchronize(st));
       s = *hslot();
+      if (guard && s.numIter >= PP.savIter && s.est < s.est_old)  // :234
+        FX_HIP(hipMemcpyAsync(safe, x, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
       resAbs = s.resAbs;
       const double rj = s.rj;
       const double resRel = resAbs / denAbs;
@@ -621,7 +634,8 @@ struct GmRun {
       o.owned.push_back(rv);
       res->normInf = o.norminf(rv);
     }
-    FX_HIP(hipMemcpyAsync(x_user, x, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
+    const VT* xout = (guard && s.numIter > PP.savIter) ? safe : x;  // :277
+    FX_HIP(hipMemcpyAsync(x_user, xout, sizeof(VT) * n, hipMemcpyDeviceToDevice, st));
     const uint32_t hc = s.histCount < W.hist_cap ? s.histCount : W.hist_cap;
     S->hist_host.assign(hc, 0.0);
     if (hc) FX_HIP(hipMemcpy(S->hist_host.data(), W.hist, sizeof(double) * hc, hipMemcpyDeviceToHost));

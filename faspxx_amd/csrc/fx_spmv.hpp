@@ -720,10 +720,16 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
 }
 
 // Functors for the common cases.
+#ifndef FX_NT_GATHER
+#define FX_NT_GATHER 0
+#endif
 template <class VT>
 struct XLoad {
   const VT* __restrict__ x;
-  __device__ __forceinline__ VT operator()(uint32_t j) const { return x[j]; }
+  __device__ __forceinline__ VT operator()(uint32_t j) const {
+    if constexpr (FX_NT_GATHER != 0) return __builtin_nontemporal_load(&x[j]);
+    else return x[j];
+  }
 };
 struct BNone {
   __device__ __forceinline__ double operator()(uint32_t) const { return 0.0; }

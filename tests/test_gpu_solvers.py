@@ -68,6 +68,45 @@ def test_cg_parity(ctx, oracle, problems, cid, scale, pc, w, force_ops):
     assert k > 10 and rel_err(hist[:k], o["hist"][:k]) < 1e-10
 
 
+@pytest.mark.parametrize("force_ops", [False, True])
+@pytest.mark.parametrize("cid,scale,sweeps,w", [(2, 32, 2, 1.0), (1, 64, 3, 0.8),
+                                                (4, 16384, 2, 1.0), (5, 64, 3, 1.0)])
+def test_cg_multisweep_jacobi(ctx, oracle, problems, cid, scale, sweeps, w, force_ops):
+    """Jacobi preconditioner of several sweeps z += D^-1 (r - A z) from z = 0
+    (src/Iter.cxx:66-77 with pcdIter > 1, example/TestPCG.cxx:71-76): the
+    fused CG forms z1 = D^-1 r inside its update and each further sweep as
+    one SpMV with the next sweep's max-norm; same bars as the one-sweep CG."""
+    pr = problems(cid, scale)
+    M = Matrix.of(pr.A)
+    o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc="jacobi", weight=w, pc_iters=sweeps,
+                     hist_cap=4000, **PRM)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "jacobi", A, w, sweeps)
+    s = fx.Solver(ctx, fx.CG)
+    s.Setup(A, P, fx.params(**PRM))
+    x = dev(ctx, np.zeros(M.nrow))
+    old = os.environ.get("FASPXX_FX_NO_FAST")
+    if force_ops:
+        os.environ["FASPXX_FX_NO_FAST"] = "1"
+    try:
+        res = s.Solve(dev(ctx, pr.b), x)
+    finally:
+        if force_ops:
+            if old is None:
+                del os.environ["FASPXX_FX_NO_FAST"]
+            else:
+                os.environ["FASPXX_FX_NO_FAST"] = old
+    assert s.last_path == ("ops" if force_ops else "fused")
+    x, hist = x.numpy(), s.history()
+    assert res.retcode == o["retcode"] == 0
+    assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+    assert rel_err(x, o["x"]) < 1e-7
+    assert np.linalg.norm(pr.b - pr.A.matvec(x)) / np.linalg.norm(pr.b) < 1e-8
+    assert abs(res.norm2 - o["norm2"]) <= 1e-3 * o["norm2"] + 1e-14
+    k = min(len(hist), 30)
+    assert k > 5 and rel_err(hist[:k], o["hist"][:k]) < 1e-10
+
+
 def test_cg_x0_and_defaults(ctx, oracle, problems):
     pr = problems(1, 64)
     M = Matrix.of(pr.A)
@@ -454,3 +493,45 @@ def test_ir_headline_configuration(ctx, problems):
     # the error against x* (b = A x*) is the FP64 solve's, to within a factor 2
     e_ir, e_64 = rel_err(xr, pr.xstar), rel_err(x64, pr.xstar)
     assert e_ir < 2 * e_64 + 1e-12, (e_ir, e_64)
+
+
+@pytest.mark.parametrize("kind,okind", [(fx.GMRES, GMRES), (fx.FGMRES, FGMRES)])
+@pytest.mark.parametrize("savIter", [10, 5000])
+def test_gmres_safeguard_fused(ctx, oracle, problems, kind, okind, savIter):
+    """The safe-guard copy (savIter <= maxIter, src/GMRES.cxx:234,277,
+    src/FGMRES.cxx:224,268) on the fused cycle graphs: the reference's test
+    after each cycle never holds (resAbsOld = resAbs after every inner
+    step), so a solve that runs past savIter returns its initial guess, and
+    one that stops before it returns its solution.  Same count and x as the
+    oracle and the per-operation path."""
+    pr = problems(3, 32)
+    prm = dict(maxIter=8000, relTol=1e-8, absTol=0.0, restart=20, savIter=savIter)
+    M = Matrix.of(pr.A)
+    x0 = np.random.default_rng(9).uniform(-1, 1, M.nrow)
+    o = oracle.solve(okind, M, pr.b, x0, pc="jacobi", **prm)
+    outs = []
+    for force_ops in (False, True):
+        A = fx.Csr.from_problem(ctx, pr.A)
+        P = fx.Precond(ctx, "jacobi", A, 1.0, 1)
+        s = fx.Solver(ctx, kind)
+        s.Setup(A, P, fx.params(**prm))
+        x = dev(ctx, x0)
+        old = os.environ.get("FASPXX_FX_NO_FAST")
+        if force_ops:
+            os.environ["FASPXX_FX_NO_FAST"] = "1"
+        try:
+            res = s.Solve(dev(ctx, pr.b), x)
+        finally:
+            if force_ops:
+                if old is None:
+                    del os.environ["FASPXX_FX_NO_FAST"]
+                else:
+                    os.environ["FASPXX_FX_NO_FAST"] = old
+        assert s.last_path == ("ops" if force_ops else "fused")
+        outs.append((res, x.numpy()))
+    for res, x in outs:
+        assert abs(res.numIter - o["numIter"]) <= 2, (res.numIter, o["numIter"])
+        if o["numIter"] > savIter:
+            assert np.array_equal(x, x0) and np.array_equal(o["x"], x0)
+        else:
+            assert rel_err(x, o["x"]) < 1e-6

@@ -138,6 +138,9 @@ SOLVE_CASES = [
     (BICGSTAB, 3, 48, "jacobi", {}),
     (BICGSTAB, 3, 48, "identity", {}),
     (BICGSTAB, 5, 24, "jacobi", {}),
+    (CG, 2, 16, "jacobi", {"pc_iters": 2}),  # pcdIter sweeps (example/TestPCG.cxx:71-76)
+    (CG, 1, 64, "jacobi", {"pc_iters": 3, "weight": 0.8}),
+    (CG, 4, 2048, "jacobi", {"pc_iters": 2}),
 ]
 
 
