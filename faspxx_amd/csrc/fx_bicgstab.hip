@@ -468,6 +468,7 @@ struct BiRun {
   const VT* dinv;  // null: identity preconditioner
   CgPrm P;
   hipStream_t st;
+  fx_op op = nullptr;  // a callback operator in place of A
 
   BiState* dstate(int c = 0) const { return (BiState*)W.state + (c & 1); }
   BiState* hslot(int slot) const { return (BiState*)W.hstate + slot; }
@@ -485,19 +486,19 @@ struct BiRun {
   int launch_iteration_t(int c, BiState* hout) {
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid_of<VT>(A, cus);
+    const unsigned gs = fused_grid<VT>(A, op, n, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
     BiSide* side = (BiSide*)W.side;
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{vp()}, BNone{},
-                                               EpiBiP<VT, JAC>{vpt(), dinv, vr0s(), dstate(c)},
-                                               part13(), nullptr, nullptr, false)));
+    FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{vp()}, BNone{},
+                                          EpiBiP<VT, JAC>{vpt(), dinv, vr0s(), dstate(c)},
+                                          part13())));
     hipLaunchKernelGGL((k_bi_s<VT>), dim3(g), dim3(kWideThreads), 0, st, vs(), (const VT*)vr(),
                        (const VT*)vpt(), n, (const BiState*)dstate(c), (const double*)part13(), gs,
                        side);
     FX_HIP(hipGetLastError());
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 0u>(
-        A, st, XLoad<VT>{vs()}, BNone{}, EpiBiS<VT, JAC>{vt(), dinv, vs(), dstate(c), side},
-        part13(), nullptr, nullptr, false)));
+    FX_TRY((fused_spmv<VT, ROW_AX, 2, 0u>(
+        A, op, W, st, XLoad<VT>{vs()}, BNone{},
+        EpiBiS<VT, JAC>{vt(), dinv, vs(), dstate(c), side}, part13())));
     hipLaunchKernelGGL((k_bi_update<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, vx(), vr(),
                        (const VT*)vp(), (const VT*)vs(), (const VT*)vt(), dinv,
                        (const VT*)vr0s(), n, (const BiState*)dstate(c), (const double*)part13(),
@@ -657,8 +658,8 @@ struct BiRun {
     s0.status = ST_RUN;
     FX_TRY(write_state(s0));
     // r = b - A x;  r0* = r;  p = r;  rr0 = (r, r0*);  loop top
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 0, 0u>(A, st, XLoad<VT>{(const VT*)vx()}, BNone{},
-                                               EpiBMinusAx<VT>{vr(), b}, nullptr)));
+    FX_TRY((fused_spmv<VT, ROW_AX, 0, 0u>(A, op, W, st, XLoad<VT>{(const VT*)vx()}, BNone{},
+                                          EpiBMinusAx<VT>{vr(), b}, nullptr)));
     const unsigned g = stream_blocks(n);
     hipLaunchKernelGGL((k_bi_init<VT>), dim3(g), dim3(kThreads), 0, st, vr0s(), vp(),
                        (const VT*)vr(), n, W.partials);
@@ -666,7 +667,7 @@ struct BiRun {
     FX_TRY((launch_finish<2, 0u>(st, W.partials, g, FinBiInit{dstate(0), P, W.hist})));
     FX_TRY(build_graph());
 
-    Ops o(W.ctx, A, S->pc, pc_dtype);
+    Ops o(W.ctx, A, op, S->pc, pc_dtype);
     fx_vec vb_ = nullptr, vx_ = nullptr, vr_ = nullptr, vp_ = nullptr, vpt_ = nullptr,
            vr0s_ = nullptr, vtmp = nullptr, vsafe = nullptr;
     const int dt = pc_dtype;
@@ -732,6 +733,7 @@ int bicgstab_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result*
   }
   BiRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
                     S->ctx->stream};
+  run.op = S->op;
   return run.run(b, x, res, S->prm, FX_F64);
 }
 

@@ -876,6 +876,7 @@ struct CgRun {
   CgPrm P;
   hipStream_t st;
   uint32_t sweeps = 1;  // Jacobi preconditioner sweeps (z buffers W.s, W.t when > 1)
+  fx_op op = nullptr;   // a callback operator in place of A
 
   // state buffer of parity c (iteration c of a replay reads c & 1)
   CgState* dstate(int c = 0) const { return (CgState*)W.state + (c & 1); }
@@ -890,10 +891,9 @@ struct CgRun {
     VT* r = (VT*)W.r;
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
-                                               EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
-                                               nullptr, false)));
-    const unsigned g1 = spmv_grid_of<VT>(A, cus);
+    FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{p}, BNone{},
+                                          EpiCgQ<VT>{q, p, dstate(c)}, part1())));
+    const unsigned g1 = fused_grid<VT>(A, op, n, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
     hipLaunchKernelGGL((k_cg_update_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, x, r, p, q,
                        dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
@@ -917,10 +917,9 @@ struct CgRun {
     VT* r = (VT*)W.r;
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{p}, BNone{},
-                                               EpiCgQ<VT>{q, p, dstate(c)}, part1(), nullptr,
-                                               nullptr, false)));
-    const unsigned g1 = spmv_grid_of<VT>(A, cus);
+    FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{p}, BNone{},
+                                          EpiCgQ<VT>{q, p, dstate(c)}, part1())));
+    const unsigned g1 = fused_grid<VT>(A, op, n, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
     hipLaunchKernelGGL((k_cg_update_w<VT, true, true>), dim3(g), dim3(kWideThreads), 0, st, x, r,
                        p, q, dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
@@ -1129,8 +1128,8 @@ struct CgRun {
     s0.status = ST_RUN;
     FX_TRY(write_state(s0));
     // r = b - A x;  z = P r;  p = z;  tmpa = (z, r);  loop top
-    FX_TRY((launch_spmv<VT, VT, ROW_AX, 0, 0u>(A, st, XLoad<VT>{(const VT*)W.x}, BNone{},
-                                               EpiBMinusW<VT>{(VT*)W.r, b}, nullptr)));
+    FX_TRY((fused_spmv<VT, ROW_AX, 0, 0u>(A, op, W, st, XLoad<VT>{(const VT*)W.x}, BNone{},
+                                          EpiBMinusW<VT>{(VT*)W.r, b}, nullptr)));
     const unsigned g = stream_blocks(n);
     if (dinv && sweeps > 1) {
       // z = P r from z = 0 with every sweep (src/CG.cxx:74-75), into W.s
@@ -1156,7 +1155,7 @@ struct CgRun {
     FX_TRY(build_graph());
 
     // Device-vector views for the host branches
-    Ops o(W.ctx, A, S->pc, pc_dtype);
+    Ops o(W.ctx, A, op, S->pc, pc_dtype);
     fx_vec vb = nullptr, vx = nullptr, vr = nullptr, vp = nullptr, vq = nullptr, vs = nullptr;
     const int dt = pc_dtype;
     Ops::chk(fx_vec_wrap(W.ctx, n, dt, (void*)b, &vb));
@@ -1404,7 +1403,16 @@ struct IrInner {
 }  // namespace
 
 bool fast_eligible(fx_solver S) {
-  if (fast_disabled() || S->A->nrow == 0) return false;
+  if (fast_disabled()) return false;
+  if (S->op) {
+    // a callback operator that only enqueues device work, with the identity
+    // or a one-sweep Jacobi preconditioner (its diagonal from its own matrix)
+    if (!S->op->device_only || S->op->nrow == 0 || S->op->nrow != S->op->mcol) return false;
+    if (S->type == FX_SOLVER_JACOBI) return false;
+    return S->pc->kind == 0 ||
+           (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A->nrow == S->op->nrow);
+  }
+  if (S->A->nrow == 0) return false;
   const bool pc_ok =
       S->pc->kind == 0 || (S->pc->kind == 1 && S->pc->iters == 1 && S->pc->A == S->A);
   // CG also takes a Jacobi preconditioner of several sweeps
@@ -1418,7 +1426,8 @@ bool fast_eligible(fx_solver S) {
 
 int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
   // a workspace is sized for one matrix: never run it on another size
-  if (S->fast && S->fast->n != S->A->nrow) S->release_fast();
+  const uint32_t n = S->op ? S->op->nrow : S->A->nrow;
+  if (S->fast && S->fast->n != n) S->release_fast();
   if (!S->fast) {
     S->fast = new FastWs();
     const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacState)
@@ -1426,8 +1435,15 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
                       : (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
                           ? gmres_state_bytes()
                                                       : sizeof(CgState);
-    int rc = ws_alloc(*S->fast, S->ctx, FX_F64, S->A->nrow, S->prm.maxIter, sb,
-                      spmv_grid_of<double>(S->A, S->ctx->sms));
+    int rc = ws_alloc(*S->fast, S->ctx, FX_F64, n, S->prm.maxIter, sb,
+                      S->op ? op_grid(n) : spmv_grid_of<double>(S->A, S->ctx->sms));
+    if (!rc && S->op) {  // the operator's work vectors, before any graph is captured
+      const size_t vb = pack_round(n) * sizeof(double);
+      if (hipMalloc(&S->fast->opy, vb) != hipSuccess || hipMalloc(&S->fast->opx, vb) != hipSuccess) {
+        set_error("hipMalloc of %zu bytes failed", vb);
+        rc = FX_ERROR_ALLOC_MEM;
+      }
+    }
     if (rc) {
       delete S->fast;
       S->fast = nullptr;
@@ -1442,6 +1458,7 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
     return gmres_fast(S, W, (const double*)b->d, (double*)x->d, res);
   CgRun<double> run{S, W, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr, make_prm(S->prm),
                     S->ctx->stream};
+  run.op = S->op;
   if (S->pc->kind == 1 && S->pc->iters > 1) {
     run.sweeps = S->pc->iters;
     const size_t vb = pack_round(W.n) * sizeof(double);

@@ -85,6 +85,8 @@ struct FastWs {
   double* dscal = nullptr;
   void* side = nullptr;        // scalars handed between the kernels of an iteration
   void* gm = nullptr;          // GMRES basis and cycle graphs (fx_gmres.hip)
+  void* opy = nullptr;         // callback operator: apply's output
+  void* opx = nullptr;         // callback operator: an operand formed before apply
   size_t part2_off = 0;        // second partials region (doubles)
   hipGraphExec_t chunk[2] = {nullptr, nullptr};  // replay graphs, one per host state slot
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -96,13 +98,109 @@ inline FastWs::~FastWs() {
   {
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
     gmres_ws_free(gm);
-    void* bufs[] = {x, x2, r, p, q, safe, s, t, partials, state, hist, dscal, side};
+    void* bufs[] = {x, x2, r, p, q, safe, s, t, partials, state, hist, dscal, side, opy, opx};
     for (void* b : bufs) (void)hipFree(b);
     if (hstate) (void)hipHostFree(hstate);
     for (auto& c : chunk)
       if (c) (void)hipGraphExecDestroy(c);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The operator of a fused solve
+// ---------------------------------------------------------------------------
+// A matrix: its SpMV with the solver's epilogue in the same kernel.  A
+// callback operator that only enqueues device work (fx_op_s::device_only):
+// apply(x) into W.opy (x formed into W.opx first when the SpMV's operand
+// functor is not a plain load), then the same epilogue over W.opy in
+// k_op_epi, whose partial sums come from op_grid(n) workgroups.  apply is
+// called while a replay graph is captured, so its work is replayed with it.
+inline unsigned op_grid(uint32_t n) { return stream_blocks(n); }
+
+template <class VT>
+inline unsigned fused_grid(fx_csr A, fx_op op, uint32_t n, int cus) {
+  return op ? op_grid(n) : spmv_grid_of<VT>(A, cus);
+}
+
+template <class XF>
+__global__ __launch_bounds__(kThreads) void k_op_operand(double* __restrict__ xs, XF xf0,
+                                                         uint32_t n) {
+  XF xf = xf0;
+  maybe_init(xf);
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) xs[i] = xf(i);
+}
+
+// The epilogue of an SpMV row by row over y = A x: v = y_i (ROW_AX) or
+// b_i - y_i (ROW_B_MINUS_AX: the operator's residual, r = b - A x, as the
+// per-operation solvers form it with apply and XPAY).
+template <int MODE, int NRED, unsigned MAXMASK, class BF, class Epi>
+__global__ __launch_bounds__(kThreads) void k_op_epi(const double* __restrict__ y, uint32_t n,
+                                                     BF bf, Epi epi0, double* partials) {
+  if (epi0.skip()) return;
+  Epi epi = epi0;
+  maybe_init(epi);
+  double acc[NRED > 0 ? NRED : 1];
+#pragma unroll
+  for (int k = 0; k < (NRED > 0 ? NRED : 1); ++k) acc[k] = 0.0;
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const double yi = y[i];
+    epi.row(i, MODE == ROW_AX ? yi : bf(i) - yi, acc);
+  }
+  if constexpr (NRED > 0) block_partials<NRED, MAXMASK>(acc, partials);
+}
+
+// W.opy = A x through the callback (x: the operand functor's values).
+template <class XF>
+int op_apply(fx_op op, FastWs& W, hipStream_t st, XF xf) {
+  const uint32_t n = W.n;
+  FX_CHECK(W.opy && W.opx, FX_ERROR_DEVICE, "operator work vectors missing");
+  const double* x = nullptr;
+  if constexpr (std::is_same_v<XF, XLoad<double>>) {
+    x = xf.x;
+  } else {
+    hipLaunchKernelGGL((k_op_operand<XF>), dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                       (double*)W.opx, xf, n);
+    FX_HIP(hipGetLastError());
+    x = (const double*)W.opx;
+  }
+  fx_vec vx = nullptr, vy = nullptr;
+  int rc = fx_vec_wrap(W.ctx, n, FX_F64, (void*)x, &vx);
+  if (!rc) rc = fx_vec_wrap(W.ctx, n, FX_F64, W.opy, &vy);
+  if (!rc) {
+    rc = op->apply(op->user, vx, vy);
+    if (rc) {
+      if (!fx_last_error()[0]) set_error("operator apply returned %d", rc);
+      rc = rc < 0 ? rc : FX_ERROR_INPUT_PAR;
+    }
+  }
+  fx_vec_destroy(vx);
+  fx_vec_destroy(vy);
+  return rc;
+}
+
+// The fused solvers' SpMV with an epilogue, on a matrix or an operator.
+template <class VT, int MODE, int NRED, unsigned MAXMASK, class XF, class BF, class Epi>
+int fused_spmv(fx_csr A, fx_op op, FastWs& W, hipStream_t st, XF xf, BF bf, Epi epi,
+               double* partials, bool finisher = false) {
+  if (!op)
+    return launch_spmv<VT, VT, MODE, NRED, MAXMASK>(A, st, xf, bf, epi, partials, nullptr,
+                                                    nullptr, finisher);
+  if constexpr (std::is_same_v<VT, double>) {
+    FX_TRY(op_apply(op, W, st, xf));
+    const unsigned g = op_grid(W.n);
+    hipLaunchKernelGGL((k_op_epi<MODE, NRED, MAXMASK, BF, Epi>), dim3(g), dim3(kThreads), 0, st,
+                       (const double*)W.opy, W.n, bf, epi, partials);
+    FX_HIP(hipGetLastError());
+    if constexpr (NRED > 0) {
+      if (finisher) return launch_finish<NRED, MAXMASK>(st, partials, g, epi);
+    }
+    return 0;
+  } else {
+    FX_CHECK(false, FX_ERROR_INPUT_PAR, "callback operators act on f64 vectors");
   }
 }
 

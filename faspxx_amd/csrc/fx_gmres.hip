@@ -430,6 +430,7 @@ struct GmRun {
   bool left;
   hipStream_t st;
   bool flex = false;  // FGMRES
+  fx_op op = nullptr;  // a callback operator in place of A
 
   GmState* dstate() const { return (GmState*)W.state; }
   VT* zvec(uint32_t j) const { return (VT*)fx_vec_data(S->Z[j]); }
@@ -458,21 +459,18 @@ struct GmRun {
   // R: residual, its norms, state to the host slot
   int launch_residual(const VT* b, const VT* x, GmState* hout) {
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid_of<VT>(A, cus);
+    const unsigned gs = fused_grid<VT>(A, op, W.n, cus);
     VT* v0 = vec(0);
     VT* tmp = (VT*)G.tmp;
     if (!left) {
-      FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 0u>(A, st, XLoad<VT>{x}, BNone{},
-                                                 EpiGmRes<VT, false, false>{v0, tmp, b, dinv},
-                                                 W.partials, nullptr, nullptr, false)));
+      FX_TRY((fused_spmv<VT, ROW_AX, 2, 0u>(A, op, W, st, XLoad<VT>{x}, BNone{},
+                                            EpiGmRes<VT, false, false>{v0, tmp, b, dinv}, W.partials)));
     } else if (dinv) {
-      FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 0u>(A, st, XLoad<VT>{x}, BNone{},
-                                                 EpiGmRes<VT, true, true>{v0, tmp, b, dinv},
-                                                 W.partials, nullptr, nullptr, false)));
+      FX_TRY((fused_spmv<VT, ROW_AX, 2, 0u>(A, op, W, st, XLoad<VT>{x}, BNone{},
+                                            EpiGmRes<VT, true, true>{v0, tmp, b, dinv}, W.partials)));
     } else {
-      FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 0u>(A, st, XLoad<VT>{x}, BNone{},
-                                                 EpiGmRes<VT, false, true>{v0, tmp, b, dinv},
-                                                 W.partials, nullptr, nullptr, false)));
+      FX_TRY((fused_spmv<VT, ROW_AX, 2, 0u>(A, op, W, st, XLoad<VT>{x}, BNone{},
+                                            EpiGmRes<VT, false, true>{v0, tmp, b, dinv}, W.partials)));
     }
     return launch_finish<2, 0u>(st, W.partials, gs, FinGmRes{dstate(), hout});
   }
@@ -480,7 +478,7 @@ struct GmRun {
   int launch_cycle(uint32_t m, const VT* b, VT* x) {
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
-    const unsigned gs = spmv_grid_of<VT>(A, cus);
+    const unsigned gs = fused_grid<VT>(A, op, W.n, cus);
     const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
     const GmLsq L = lsq();
     const GmPrm P = prm(m);
@@ -495,27 +493,24 @@ struct GmRun {
       if (flex) {
         const VT* z = vec(k);  // identity: Z[k] = V[k]
         if (dinv) {
+          // the preconditioner's sweep on its own matrix
           FX_TRY((launch_spmv<VT, VT, ROW_B_MINUS_AX, 0, 0u>(
-              A, st, XLoad<VT>{zvec(k)}, BLoad<VT>{vec(k)},
+              S->pc->A, st, XLoad<VT>{zvec(k)}, BLoad<VT>{vec(k)},
               EpiFgmPc<VT>{(VT*)G.tmp, zvec(k), vec(k), dinv, G.zinf + k, cst}, nullptr, nullptr,
               nullptr, false)));
           z = (const VT*)G.tmp;
         }
-        FX_TRY((launch_spmv<VT, VT, ROW_AX, 2, 2u>(A, st, XLoad<VT>{z}, BNone{},
-                                                   EpiFgmW<VT>{w, zvec(k), z, vec(0), cst}, pa,
-                                                   nullptr, nullptr, false)));
+        FX_TRY((fused_spmv<VT, ROW_AX, 2, 2u>(A, op, W, st, XLoad<VT>{z}, BNone{},
+                                              EpiFgmW<VT>{w, zvec(k), z, vec(0), cst}, pa)));
       } else if (!left && dinv) {
-        FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XScaled<VT>{vec(k), dinv}, BNone{},
-                                                   EpiGmW<VT, false>{w, vec(0), dinv, cst}, pa,
-                                                   nullptr, nullptr, false)));
+        FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XScaled<VT>{vec(k), dinv}, BNone{},
+                                              EpiGmW<VT, false>{w, vec(0), dinv, cst}, pa)));
       } else if (left && dinv) {
-        FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{vec(k)}, BNone{},
-                                                   EpiGmW<VT, true>{w, vec(0), dinv, cst}, pa,
-                                                   nullptr, nullptr, false)));
+        FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{vec(k)}, BNone{},
+                                              EpiGmW<VT, true>{w, vec(0), dinv, cst}, pa)));
       } else {
-        FX_TRY((launch_spmv<VT, VT, ROW_AX, 1, 0u>(A, st, XLoad<VT>{vec(k)}, BNone{},
-                                                   EpiGmW<VT, false>{w, vec(0), dinv, cst}, pa,
-                                                   nullptr, nullptr, false)));
+        FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{vec(k)}, BNone{},
+                                              EpiGmW<VT, false>{w, vec(0), dinv, cst}, pa)));
       }
       const double* pin = pa;
       unsigned gin = gs;
@@ -628,7 +623,7 @@ struct GmRun {
     res->numIter = s.numIter;
     if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
       res->norm2 = left ? s.rj : resAbs;  // the norm of tmp (left) or V[0] (right), :269-272
-      Ops o(W.ctx, A, S->pc, FX_F64);
+      Ops o(W.ctx, A, op, S->pc, FX_F64);
       fx_vec rv = nullptr;
       Ops::chk(fx_vec_wrap(W.ctx, n, FX_F64, left ? G.tmp : G.V[0], &rv));
       o.owned.push_back(rv);
@@ -690,6 +685,7 @@ int gmres_fast(fx_solver S, FastWs& W, const double* b, double* x, fx_result* re
   }
   GmRun<double> run{S, W, GG, S->A, S->pc->kind == 1 ? S->pc->dinv : nullptr,
                     !flex && S->prm.leftPrecond != 0, S->ctx->stream, flex};
+  run.op = S->op;
   return run.run(b, x, res);
 }
 

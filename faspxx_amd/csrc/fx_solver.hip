@@ -57,6 +57,9 @@ Ops::Ops(fx_ctx c, fx_csr a, fx_pc p, int dtype) : ctx(c), A(a), pc(p), dt(dtype
 Ops::Ops(fx_ctx c, fx_op a, fx_pc p, int dtype)
     : ctx(c), A(nullptr), pc(p), dt(dtype), n(a->nrow), op(a) {}
 
+Ops::Ops(fx_ctx c, fx_csr a, fx_op o, fx_pc p, int dtype)
+    : ctx(c), A(o ? nullptr : a), pc(p), dt(dtype), n(o ? o->nrow : a->nrow), op(o) {}
+
 Ops::~Ops() {
   for (fx_vec v : owned) fx_vec_destroy(v);
 }
@@ -831,6 +834,13 @@ int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply,
   return 0;
 }
 
+int fx_op_set_device_only(fx_op op, int device_only) {
+  clear_error();
+  FX_CHECK(op, FX_ERROR_INPUT_PAR, "null argument");
+  op->device_only = device_only != 0;
+  return 0;
+}
+
 int fx_op_destroy(fx_op op) {
   delete op;
   return 0;
@@ -890,7 +900,7 @@ int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
   FX_HIP(hipEventRecord(e0, s->ctx->stream));
   int rc = 0;
   try {
-    s->last_path = (!s->op && fast_eligible(s)) ? FX_PATH_FUSED : FX_PATH_OPS;
+    s->last_path = fast_eligible(s) ? FX_PATH_FUSED : FX_PATH_OPS;
     if (s->last_path == FX_PATH_FUSED) {
       rc = fast_solve(s, b, x, res);
     } else {

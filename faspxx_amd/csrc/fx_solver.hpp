@@ -17,6 +17,9 @@ struct fx_op_s {
   uint32_t nrow = 0, mcol = 0;
   fx_op_apply_fn apply = nullptr;
   void* user = nullptr;
+  // apply only enqueues device work on the context's stream, the same work
+  // on every call: the fused solvers capture it in their replay graphs
+  bool device_only = false;
 };
 
 struct fx_solver_s {
@@ -60,6 +63,8 @@ struct Ops {
 
   Ops(fx_ctx c, fx_csr a, fx_pc p, int dtype);
   Ops(fx_ctx c, fx_op a, fx_pc p, int dtype);
+  // the operator is op when it is set, else A
+  Ops(fx_ctx c, fx_csr a, fx_op op, fx_pc p, int dtype);
   Ops(const Ops&) = delete;
   Ops& operator=(const Ops&) = delete;
   ~Ops();

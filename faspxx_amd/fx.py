@@ -132,6 +132,7 @@ _SIGS = {
     "fx_csr_shift_diag": (C.c_int, [_P, _P, C.c_double]),
     "fx_op_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(_P)]),
     "fx_op_destroy": (C.c_int, [_P]),
+    "fx_op_set_device_only": (C.c_int, [_P, C.c_int]),
     "fx_solver_setup_op": (C.c_int, [_P, _P, _P, _P]),
     "fx_csr_pattern_grad": (C.c_int, [_P, _P, _P, _P, _P]),
     "fx_pc_create_identity": (C.c_int, [_P, C.POINTER(_P)]),
@@ -623,9 +624,14 @@ class _View(Vec):
 class Operator:
     """A linear operator given by its action, LOP with a user Apply
     (src/LOP.hxx:56-59): apply(x: Vec, y: Vec) computes y = A x on the
-    context's stream.  An exception in apply becomes ERROR_INPUT_PAR."""
+    context's stream.  An exception in apply becomes ERROR_INPUT_PAR.
 
-    def __init__(self, ctx: Context, n: int, apply):
+    device_only=True declares that apply only enqueues device work on the
+    context's stream, the same work on every call (fx_op_set_device_only):
+    the solvers then run their fused form and capture apply's work in their
+    replay graphs."""
+
+    def __init__(self, ctx: Context, n: int, apply, device_only: bool = False):
         self.ctx, self.n, self._apply = ctx, n, apply
         self.error = None
 
@@ -641,6 +647,8 @@ class Operator:
         h = C.c_void_p()
         _chk(load().fx_op_create(ctx.h, n, n, C.cast(self._fn, C.c_void_p), None, C.byref(h)))
         self.h = h
+        if device_only:
+            _chk(load().fx_op_set_device_only(h, 1))
         ctx._adopt(self)
 
     def close(self):

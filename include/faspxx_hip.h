@@ -325,6 +325,16 @@ int fx_solver_clean(fx_solver s);
  * solver needs a matrix (its diagonal), as in the reference. */
 int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply, void* user,
                  fx_op* out);
+/* Declares that op's apply only enqueues device work on the context's stream
+ * (no host synchronisation, allocation or host-side reads of device data)
+ * and enqueues the same work on every call, e.g. a device stencil kernel
+ * for the MatFree::Apply of example/TestMatFree.cxx:86-176.  The Krylov
+ * solvers then run their fused form on it (the identity or a one-sweep
+ * Jacobi preconditioner): apply is called while the replay graph of the
+ * first Solve after fx_solver_setup_op / fx_solver_update_op is captured,
+ * and its device work is replayed from then on.  No counterpart in the
+ * reference, whose LOP::Apply is a host call. */
+int fx_op_set_device_only(fx_op op, int device_only);
 int fx_op_destroy(fx_op op);
 int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm);
 /* fx_solver_update for an operator (see fx_solver_update). */

@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes as C
 import pathlib
 import subprocess
+import tempfile
 
 import numpy as np
 
@@ -450,8 +451,21 @@ class Reference:
                          _p(L.va), R.mcol, R.nnz, _p(R.rp), _p(R.ci), _p(R.va))
 
     def _read(self, fn, path):
+        path = pathlib.Path(path)
         o = _RefMatOut()
-        rc = fn(str(path).encode(), C.byref(o))
+        if path.suffix.lower() == ".csr" and path.exists():
+            # ReadCSR loads the file into a buffer of its exact length and
+            # scans the values for a NUL that the buffer does not hold
+            # (src/ReadData.cxx:243-251,331-345), so it reads past the
+            # allocation and its result depends on the heap there.  It is
+            # given a copy of the file with a NUL appended: the scan then
+            # ends inside the buffer, after the file's own last line.
+            with tempfile.TemporaryDirectory() as td:
+                tmp = pathlib.Path(td) / path.name
+                tmp.write_bytes(path.read_bytes() + b"\0")
+                rc = fn(str(tmp).encode(), C.byref(o))
+        else:
+            rc = fn(str(path).encode(), C.byref(o))
         if rc != 0:
             return rc
         try:

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from faspxx_amd import fx
-from oracle import CG, Matrix
+from oracle import BICGSTAB, CG, FGMRES, GMRES, Matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +56,70 @@ def test_operator_equals_matrix_solve(ctx, problems, kind, pc):
     assert r1.numIter == r0.numIter
     assert np.array_equal(x1, x0)
     assert len(calls) >= r1.numIter
+
+
+OKINDS = {fx.CG: CG, fx.BICGSTAB: BICGSTAB, fx.GMRES: GMRES, fx.FGMRES: FGMRES}
+
+
+@pytest.mark.parametrize("kind", [fx.CG, fx.BICGSTAB, fx.GMRES, fx.FGMRES])
+@pytest.mark.parametrize("pc", ["identity", "jacobi"])
+def test_device_only_operator_fused(ctx, oracle, problems, kind, pc):
+    """An operator whose apply only enqueues device work (here the matrix's
+    own SpMV, fx_op_set_device_only) runs the fused solvers: its work is
+    captured in the replay graphs (CG and BiCGStab call apply far fewer
+    times than they iterate).  Against the per-operation operator solve and
+    the oracle's solve on the matrix: count +-2, x to 1e-6 relative (FGMRES
+    with Jacobi: the 200-iteration budget, its residual norm to 1e-6)."""
+    pr = problems(3, 64) if kind != fx.CG else problems(5, 64)
+    n = pr.A.nrow
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, pc, A, 1.0, 1)
+    flex_budget = kind == fx.FGMRES and pc == "jacobi"
+    prm = dict(maxIter=200, relTol=1e-12, absTol=0.0) if flex_budget else \
+        dict(maxIter=3000, relTol=1e-8, absTol=0.0)
+    b = fx.Vec.from_numpy(ctx, pr.b)
+    calls = []
+
+    def apply(x, y):
+        calls.append(1)
+        A.Apply(x, y)
+
+    def solve():
+        s = fx.Solver(ctx, kind)
+        s.Setup(fx.Operator(ctx, n, apply, device_only=True), P, fx.params(**prm))
+        x = fx.Vec(ctx, n)
+        r = s.Solve(b, x)
+        return r, x.numpy(), s.last_path
+
+    r1, x1, path = solve()
+    assert path == "fused"
+    if kind in (fx.CG, fx.BICGSTAB):
+        assert len(calls) < r1.numIter, (len(calls), r1.numIter)
+    r0, x0, path0 = per_operation(solve)
+    assert path0 == "ops"
+    o = oracle.solve(OKINDS[kind], Matrix.of(pr.A), pr.b, np.zeros(n), pc=pc, **prm)
+    for r, x in ((r1, x1), (r0, x0)):
+        assert r.retcode == o["retcode"]
+        assert abs(r.numIter - o["numIter"]) <= 2, (r.numIter, o["numIter"])
+        if flex_budget:
+            assert abs(r.norm2 - o["norm2"]) <= 1e-6 * o["norm2"]
+        else:
+            assert np.linalg.norm(x - o["x"]) <= 1e-6 * np.linalg.norm(o["x"])
+
+
+def test_device_only_operator_error(ctx, problems):
+    pr = problems(1, 16)
+    n = pr.A.nrow
+
+    def boom(x, y):
+        raise RuntimeError("apply failed")
+
+    op = fx.Operator(ctx, n, boom, device_only=True)
+    s = fx.Solver(ctx, fx.CG)
+    s.Setup(op, fx.Precond(ctx, "identity", None, 1.0, 1), fx.params(maxIter=10, relTol=1e-8))
+    with pytest.raises(fx.FxError, match="ERROR_INPUT_PAR"):
+        s.Solve(fx.Vec.from_numpy(ctx, pr.b), fx.Vec(ctx, n))
+    assert isinstance(op.error, RuntimeError)
 
 
 def stencil_apply(m):
