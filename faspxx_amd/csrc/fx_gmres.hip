@@ -213,7 +213,36 @@ __device__ __forceinline__ void triangular_solve(const GmLsq& L, uint32_t cnt) {
   }
 }
 
-// G_k: t = ||w||, w *= 1/t, H(k+1,k) = t, Givens, stopping test.
+// The same Givens update and triangular solve on LDS copies (one thread;
+// every dependent step then costs LDS latency instead of a global-memory
+// round trip): columns of up to kGivLds rows, triangles up to kTriLds.
+constexpr uint32_t kGivLds = 512;
+constexpr uint32_t kTriLds = 64;
+
+__device__ __forceinline__ void givens_column_lds(double* __restrict__ col,
+                                                  double* __restrict__ hc,
+                                                  double* __restrict__ hs, double& gc1,
+                                                  double& gcnt, uint32_t cnt) {
+  const uint32_t c1 = cnt - 1;
+  for (uint32_t jj = 1; jj < cnt; ++jj) {
+    const double t = col[jj - 1];
+    col[jj - 1] = fma(hs[jj - 1], col[jj], hc[jj - 1] * t);
+    col[jj] = fma(-hs[jj - 1], t, hc[jj - 1] * col[jj]);
+  }
+  double t = col[cnt] * col[cnt];
+  t = fma(col[c1], col[c1], t);
+  t = sqrt(t);
+  const double gamma = t > kSmallDbl ? t : kSmallDbl;
+  hc[c1] = col[c1] / gamma;
+  hs[c1] = col[cnt] / gamma;
+  col[c1] = fma(hs[c1], col[cnt], hc[c1] * col[c1]);
+  gcnt = -hs[c1] * gc1;
+  gc1 = hc[c1] * gc1;
+}
+
+// G_k: t = ||w||, w *= 1/t, H(k+1,k) = t, Givens, stopping test (workgroup
+// 0; it stages column k, the rotations and g[k] in LDS while every
+// workgroup scales w), and at the end of the cycle the triangular solve.
 template <class VT>
 __global__ __launch_bounds__(kWideThreads) void k_gm_givens(
     VT* __restrict__ w, uint32_t n, GmState* st, const double* __restrict__ pin, unsigned gin,
@@ -224,29 +253,84 @@ __global__ __launch_bounds__(kWideThreads) void k_gm_givens(
   partials_combine<1, 0u, kWideThreads>(t2);
   const double t = sqrt(t2[0]);  // VEC::Norm2
   const double a = 1.0 / t;      // src/GMRES.cxx:177-178 (t > SMALL_DBL always holds)
+  const uint32_t cnt = k + 1;
+  const bool staged = cnt < kGivLds;
+  __shared__ double col[kGivLds + 1], shc[kGivLds], shs[kGivLds], sg[2];
+  __shared__ double tri[kTriLds * kTriLds], tg[kTriLds];
+  __shared__ int s_end;
+  if (blockIdx.x == 0 && staged) {
+    for (uint32_t i = threadIdx.x; i <= cnt; i += kWideThreads)
+      col[i] = i == cnt ? t : L.H[(size_t)i * L.ld + k];
+    for (uint32_t i = threadIdx.x; i < k; i += kWideThreads) {
+      shc[i] = L.hc[i];
+      shs[i] = L.hs[i];
+    }
+    if (threadIdx.x == 0) sg[0] = L.g[k];
+  }
   const uint32_t stride = gridDim.x * kWideThreads;
   for (uint32_t i = blockIdx.x * kWideThreads + threadIdx.x; i < n; i += stride)
     w[i] = w[i] * (VT)a;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x != 0) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
     GmState S = *st;
-    const uint32_t cnt = k + 1;
-    L.H[(size_t)cnt * L.ld + k] = t;
     S.numIter += 1;
     S.count = cnt;
-    givens_column(L, cnt);
-    S.resAbs = fabs(L.g[cnt]);
+    double gcnt;
+    if (staged) {
+      double gk = sg[0];
+      givens_column_lds(col, shc, shs, gk, gcnt, cnt);
+      L.hc[k] = shc[k];
+      L.hs[k] = shs[k];
+      L.g[k] = gk;
+      L.g[cnt] = gcnt;
+      sg[0] = gk;
+    } else {
+      L.H[(size_t)cnt * L.ld + k] = t;
+      givens_column(L, cnt);
+      gcnt = L.g[cnt];
+    }
+    S.resAbs = fabs(gcnt);
     S.est = S.resAbs;
     S.est_old = S.resAbs;  // resAbsOld = resAbs after the ratio, :198-201
     const double resRel = S.resAbs / S.denAbs;
     const bool conv = (S.resAbs < P.absTol || resRel < P.relTol) && S.numIter > P.minIter;
-    if (conv || cnt >= P.m || S.numIter >= P.maxIter) {
+    const bool end = conv || cnt >= P.m || S.numIter >= P.maxIter;
+    if (end) {
       S.status = GM_CYCLE_END;
-      triangular_solve(L, cnt);
     } else if (S.numIter >= P.minIter && hist) {  // the next iteration's record, :153
       hist[S.numIter] = S.resAbs;
       S.histCount = S.numIter + 1;
     }
+    s_end = end;
     *st = S;
+  }
+  __syncthreads();
+  if (staged)  // the rotated column back to H
+    for (uint32_t i = threadIdx.x; i <= cnt; i += kWideThreads) L.H[(size_t)i * L.ld + k] = col[i];
+  if (!s_end) return;
+  if (staged && cnt <= kTriLds) {  // the cycle's y = H^-1 g on LDS copies
+    for (uint32_t e = threadIdx.x; e < cnt * cnt; e += kWideThreads) {
+      const uint32_t i = e / cnt, j = e % cnt;
+      if (j >= i) tri[i * cnt + j] = j == k ? col[i] : L.H[(size_t)i * L.ld + j];
+    }
+    for (uint32_t i = threadIdx.x; i < cnt; i += kWideThreads) tg[i] = i == k ? sg[0] : L.g[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {  // src/GMRES.cxx:213-219, as triangular_solve
+      const uint32_t c1 = cnt - 1;
+      tg[c1] = tg[c1] / tri[c1 * cnt + c1];
+      for (int kk = (int)cnt - 2; kk >= 0; --kk) {
+        double u = 0.0;
+        for (uint32_t jj = (uint32_t)kk + 1; jj < cnt; ++jj) u = fma(-tri[kk * cnt + jj], tg[jj], u);
+        u += tg[kk];
+        tg[kk] = u / tri[kk * cnt + kk];
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += kWideThreads) L.g[i] = tg[i];
+  } else {
+    __syncthreads();  // the column written back above
+    if (threadIdx.x == 0) triangular_solve(L, cnt);
   }
 }
 
