@@ -463,55 +463,111 @@ struct FinCgInit {
 
 // ---- Jacobi solver kernels (src/Iter.cxx:69-99) ---------------------------
 
-struct JacState {
-  double resAbs, resRel, denAbs, xinf;
-  uint32_t numIter, histCount, answer, pad;  // answer: 0 -> buffer A, 1 -> buffer B
+// The Jacobi solver's sweep with its stopping decision one sweep late, so
+// that no finisher kernel sits between sweeps.  Sweep m reads x_m (buffer
+// m % 3) and the flag "||x_m||_inf >= 1e-20" (flag m % 3: MAT::Residual's
+// shortcut, src/MAT.cxx:455-476), writes x_{m+1} (buffer (m+1) % 3), raises
+// flag (m+1) % 3 from every workgroup holding an entry of x_{m+1} at or
+// above 1e-20 (a flag written only with 1 needs no reduction), and writes
+// its partial sums of ||b - A x_m||^2.  Its workgroup 0 meanwhile sums sweep
+// m-1's partials and takes that sweep's decision (src/Iter.cxx:69-94) from
+// state buffer m % 2 into (m+1) % 2 (both when it stops), and clears flag
+// (m+2) % 3 for sweep m+1.  With three x buffers x_{m-1} is still intact
+// when sweep m-1 turns out to have converged; sweep m's own x_{m+1} is then
+// dropped, and later sweeps skip.
+struct JacLagState {
+  double resAbs, resRel, denAbs, pad;
+  uint32_t numIter, histCount, answer, pending;  // answer: x buffer 0..2
   int status, pad2;
 };
 
-// One sweep reading x (buffer `src`) and writing x+ (buffer `dst`):
-// work = b - A x (or b when ||x||_inf < 1e-20), ||work||, x+ = x + work .* dinv.
-struct EpiJacobi {
+struct EpiJacobiLag {
   double* __restrict__ xn;
   const double* __restrict__ x;
   const double* __restrict__ b;
   const double* __restrict__ dinv;
-  JacState* st;
+  const int* flag_in;
+  int* flag_out;
+  int* flag_clear;
+  JacLagState* s_in;
+  JacLagState* s_out;
+  const double* prev;  // sweep m-1's partials (gprev of them)
+  unsigned gprev;
   CgPrm P;
-  uint32_t self;  // index of `x` (0 A / 1 B)
+  uint32_t buf_prev, buf_cur;  // the x buffers of x_{m-1} and x_m
   double* hist;
-  __device__ bool skip() const { return st->status != ST_RUN; }
+  int use_v;  // ||x_m||_inf >= 1e-20: work = b - A x, else work = b
+  __device__ bool skip() const { return s_in->status != ST_RUN; }
+  __device__ void init() { use_v = *flag_in; }
+  __device__ void block_start() {
+    if (blockIdx.x != 0) return;
+    if (!s_in->pending) {  // the first sweep: nothing to decide yet
+      if (threadIdx.x == 0) {
+        JacLagState S = *s_in;
+        S.pending = 1;
+        *s_out = S;
+        *flag_clear = 0;
+      }
+      return;
+    }
+    double acc[1];
+    partials_thread<1, 0u, kThreads>(prev, gprev, threadIdx.x, acc);
+    partials_combine<1, 0u, kThreads>(acc);
+    if (threadIdx.x != 0) return;
+    JacLagState S = *s_in;
+    const uint32_t it = S.numIter;  // the sweep being decided (m - 1)
+    *flag_clear = 0;
+    if (it >= P.minIter) {
+      const double resAbs = sqrt(acc[0]);
+      if (it == P.minIter) S.denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+      S.resAbs = resAbs;
+      S.resRel = resAbs / S.denAbs;
+      if (S.resRel < P.relTol || resAbs < P.absTol) {
+        S.status = ST_DONE;
+        S.answer = buf_prev;  // x was not updated in sweep m-1
+        *s_out = S;
+        *s_in = S;
+        return;
+      }
+      hist[it] = resAbs;
+      S.histCount = it + 1;
+    }
+    S.numIter = it + 1;
+    S.answer = buf_cur;
+    if (S.numIter >= P.maxIter) {
+      S.status = ST_DONE;
+      *s_in = S;
+    }
+    *s_out = S;
+  }
   __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
-    const double w = (st->xinf < kCloseZero) ? b[i] : v;
+    const double w = use_v ? v : b[i];
     acc[0] = fma(w, w, acc[0]);
     const double wd = w * dinv[i];  // work.PointwiseMult(diagInv)
     const double xv = x[i] + wd;    // x += work
     xn[i] = xv;
     acc[1] = fmax(acc[1], fabs(xv));
   }
-  __device__ void fin(double (&s)[2]) const {
-    if (threadIdx.x != 0) return;
-    JacState* S = st;
-    const uint32_t it = S->numIter;
-    if (it >= P.minIter) {
-      const double resAbs = sqrt(s[0]);
-      if (it == P.minIter) S->denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
-      S->resAbs = resAbs;
-      S->resRel = resAbs / S->denAbs;
-      if (S->resRel < P.relTol || resAbs < P.absTol) {
-        S->status = ST_DONE;
-        S->answer = self;  // x was not updated in this sweep
-        return;
-      }
-      hist[it] = resAbs;
-      S->histCount = it + 1;
-    }
-    S->numIter = it + 1;
-    S->xinf = s[1];
-    S->answer = 1u - self;
-    if (S->numIter >= P.maxIter) S->status = ST_DONE;
+  __device__ void block_end(double (&acc)[2]) const {
+    const int any = __syncthreads_or(acc[1] >= kCloseZero);
+    if (threadIdx.x == 0 && any) *flag_out = 1;
   }
+  __device__ void fin(double (&)[2]) const {}
 };
+
+// The state after a replay into a pinned host slot (no copy enqueued
+// between replays).
+__global__ void k_jac_report(const JacLagState* __restrict__ s, JacLagState* hout) {
+  if (threadIdx.x == 0) *hout = *s;
+}
+
+__global__ void k_jac_flags(int* flags, const double* xinf) {
+  flags[0] = (*xinf < kCloseZero) ? 0 : 1;
+  flags[1] = 0;
+  flags[2] = 0;
+}
+
+constexpr int kJacChunk = 6;  // sweeps per replay: a multiple of 3 buffers and 2 states
 
 // Final residual norms (src/Iter.cxx:102-109): ||b - A x||_2 and _inf.
 struct EpiResNorms {
@@ -1228,71 +1284,80 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   hipStream_t st = W.ctx->stream;
   fx_csr A = S->A;
   const uint32_t n = W.n;
-  JacState* ds = (JacState*)W.state;
-  double* xa = (double*)W.x;
-  double* xb = (double*)W.x2;
-  FX_HIP(hipMemcpyAsync(xa, x_user, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-  JacState s0{};
+  JacLagState* ds = (JacLagState*)W.state;  // two parity buffers
+  double* xbuf[3] = {(double*)W.x, (double*)W.x2, (double*)W.r};
+  int* flags = (int*)W.side;
+  const unsigned G = spmv_grid_of<double>(A, W.ctx->sms);
+  double* part[2] = {W.partials, W.partials + 2 * (size_t)G};
+  FX_HIP(hipMemcpyAsync(xbuf[0], x_user, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  JacLagState s0{};
   s0.resAbs = 1.0;
   s0.resRel = 1.0;
   s0.denAbs = 1.0;
   s0.status = P.maxIter == 0 ? ST_DONE : ST_RUN;
-  FX_HIP(hipMemcpyAsync(ds, &s0, sizeof(JacState), hipMemcpyHostToDevice, st));
-  FX_TRY(norminf_to(st, xa, n, &ds->xinf, W.partials));
+  FX_HIP(hipMemcpyAsync(ds, &s0, sizeof(JacLagState), hipMemcpyHostToDevice, st));
+  FX_TRY(norminf_to(st, xbuf[0], n, W.dscal, W.partials));
+  hipLaunchKernelGGL(k_jac_flags, dim3(1), dim3(1), 0, st, flags, (const double*)W.dscal);
+  FX_HIP(hipGetLastError());
   const double* dinv = S->pc->dinv;
-  auto sweeps = [&]() -> int {
-    for (int c = 0; c < kChunk; ++c) {
-      const bool even = (c % 2) == 0;
-      const double* src = even ? xa : xb;
-      double* dst = even ? xb : xa;
+  auto sweeps = [&](int slot) -> int {
+    for (int c = 0; c < kJacChunk; ++c) {
+      const EpiJacobiLag e{xbuf[(c + 1) % 3], xbuf[c % 3], b, dinv, flags + c % 3,
+                           flags + (c + 1) % 3, flags + (c + 2) % 3, ds + c % 2, ds + (c + 1) % 2,
+                           part[(c + 1) % 2], G, P, (uint32_t)((c + 2) % 3), (uint32_t)(c % 3),
+                           W.hist, 0};
       FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
-          A, st, XLoad<double>{src}, BLoad<double>{b},
-          EpiJacobi{dst, src, b, dinv, ds, P, even ? 0u : 1u, W.hist}, W.partials)));
+          A, st, XLoad<double>{xbuf[c % 3]}, BLoad<double>{b}, e, part[c % 2], nullptr,
+          nullptr, false)));
     }
+    hipLaunchKernelGGL(k_jac_report, dim3(1), dim3(64), 0, st, (const JacLagState*)ds,
+                       (JacLagState*)W.hstate + slot);
+    FX_HIP(hipGetLastError());
     return 0;
   };
   const bool graph = st != nullptr;  // the legacy null stream cannot be captured
   if (graph && !W.chunk[0]) {
-    hipGraph_t g = nullptr;
-    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const int rc = sweeps();
-    hipError_t e = hipStreamEndCapture(st, &g);
-    if (rc) return rc;
-    FX_HIP(e);
-    e = hipGraphInstantiate(&W.chunk[0], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    FX_HIP(e);
-  }
-  auto replay = [&]() -> int {
-    if (graph) {
-      FX_HIP(hipGraphLaunch(W.chunk[0], st));
-      return 0;
+    for (int slot = 0; slot < 2; ++slot) {
+      hipGraph_t g = nullptr;
+      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = sweeps(slot);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      FX_HIP(e);
+      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      FX_HIP(e);
     }
-    return sweeps();
-  };
-  // kChunk is even: every replay starts reading buffer A
-  JacState s{};
-  int cur = 0;
-  for (int i = 0; i < 2; ++i) {
-    FX_TRY(replay());
-    FX_HIP(hipMemcpyAsync((JacState*)W.hstate + i, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
-    FX_HIP(hipEventRecord(W.ev[i], st));
   }
-  const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
+  auto replay = [&](int slot) -> int {
+    if (graph) {
+      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
+    } else {
+      FX_TRY(sweeps(slot));
+    }
+    FX_HIP(hipEventRecord(W.ev[slot], st));
+    return 0;
+  };
+  // kJacChunk is a multiple of 6: every replay starts on x buffer 0 and
+  // state buffer 0, and ends with the state in buffer 0, which its last
+  // kernel copies to the replay's host slot
+  JacLagState s{};
+  int cur = 0;
+  for (int i = 0; i < 2; ++i) FX_TRY(replay(i));
+  const uint64_t max_chunks = (uint64_t)PP.maxIter / kJacChunk + 4;
   uint64_t chunks = 2;
   for (;;) {
     FX_HIP(hipEventSynchronize(W.ev[cur]));
-    if (((const JacState*)W.hstate + cur)->status != ST_RUN) break;
+    if (((const JacLagState*)W.hstate + cur)->status != ST_RUN) break;
     FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "Jacobi device loop did not stop");
     ++chunks;
-    FX_TRY(replay());
-    FX_HIP(hipMemcpyAsync((JacState*)W.hstate + cur, ds, sizeof(JacState), hipMemcpyDeviceToHost, st));
-    FX_HIP(hipEventRecord(W.ev[cur], st));
+    FX_TRY(replay(cur));
     cur ^= 1;
   }
   FX_HIP(hipStreamSynchronize(st));
-  FX_HIP(hipMemcpy(&s, ds, sizeof(JacState), hipMemcpyDeviceToHost));
-  double* ans = s.answer ? xb : xa;
+  FX_HIP(hipMemcpy(&s, ds, sizeof(JacLagState), hipMemcpyDeviceToHost));
+  FX_CHECK(s.answer < 3, FX_ERROR_DEVICE, "Jacobi answer buffer %u", s.answer);
+  double* ans = xbuf[s.answer];
   res->numIter = s.numIter;
   if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
     FX_TRY(norminf_to(st, ans, n, W.dscal, W.partials));
@@ -1430,7 +1495,7 @@ int fast_solve(fx_solver S, fx_vec b, fx_vec x, fx_result* res) {
   if (S->fast && S->fast->n != n) S->release_fast();
   if (!S->fast) {
     S->fast = new FastWs();
-    const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacState)
+    const size_t sb = S->type == FX_SOLVER_JACOBI     ? sizeof(JacLagState)
                       : S->type == FX_SOLVER_BICGSTAB ? bicgstab_state_bytes()
                       : (S->type == FX_SOLVER_GMRES || S->type == FX_SOLVER_FGMRES)
                           ? gmres_state_bytes()

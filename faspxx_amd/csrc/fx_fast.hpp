@@ -78,7 +78,7 @@ struct FastWs {
   void* s = nullptr;   // BiCGStab: s
   void* t = nullptr;   // BiCGStab: P A s
   double* partials = nullptr;
-  void* state = nullptr;  // CgState / JacState
+  void* state = nullptr;  // solver state, two parity buffers
   void* hstate = nullptr; // pinned, 2 slots
   double* hist = nullptr;
   uint32_t hist_cap = 0;

@@ -353,6 +353,29 @@ __device__ __forceinline__ void maybe_init(T& t) {
   if constexpr (has_init<T>::value) t.init();
 }
 
+// Optional workgroup hooks of SpMV epilogues, called by every thread of
+// every workgroup that did not skip: block_start() after init(), and
+// block_end(acc) with the thread's accumulators before the partial sums
+// are written.  Both may synchronise the workgroup.
+template <class T, class = void>
+struct has_block_start : std::false_type {};
+template <class T>
+struct has_block_start<T, std::void_t<decltype(std::declval<T&>().block_start())>>
+    : std::true_type {};
+template <class T>
+__device__ __forceinline__ void maybe_block_start(T& t) {
+  if constexpr (has_block_start<T>::value) t.block_start();
+}
+template <class T, class A, class = void>
+struct has_block_end : std::false_type {};
+template <class T, class A>
+struct has_block_end<T, A, std::void_t<decltype(std::declval<T&>().block_end(std::declval<A&>()))>>
+    : std::true_type {};
+template <class T, class A>
+__device__ __forceinline__ void maybe_block_end(T& t, A& acc) {
+  if constexpr (has_block_end<T, A>::value) t.block_end(acc);
+}
+
 // Finisher functor: bool skip() const; void fin(double (&sum)[K]) (all
 // threads of the finisher workgroup call it; it usually acts in thread 0);
 // optionally void begin(), called first, so that loads of solver state

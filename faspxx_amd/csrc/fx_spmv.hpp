@@ -557,6 +557,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
   Epi epi = epi0;
   maybe_init(xf);
   maybe_init(epi);
+  maybe_block_start(epi);
   __shared__ SpmvLds<MT, CF> lds;
   double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
@@ -716,6 +717,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
       }
     }
   }
+  maybe_block_end(epi, acc);
   if constexpr (NRED > 0) block_partials<NRED, MAXMASK>(acc, partials);
 }
 
