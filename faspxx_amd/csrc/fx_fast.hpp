@@ -7,7 +7,13 @@
 
 namespace fx {
 
-constexpr int kChunk = 8;  // iterations per graph replay
+// Iterations per graph replay (even: the state's parity buffers restart at
+// 0 with every replay).
+#ifndef FX_CHUNK
+#define FX_CHUNK 16
+#endif
+constexpr int kChunk = FX_CHUNK;
+static_assert(kChunk % 2 == 0, "replays hold an even number of iterations");
 
 enum { ST_RUN = 0, ST_DONE = 1, ST_HOST = 2 };  // device loop status
 enum { R_BREAKDOWN = 1, R_CHECKS = 2 };          // why the device loop stopped
@@ -52,11 +58,11 @@ constexpr int kPassPacks = FX_PASS_PACKS;
 
 // Workgroups of those kernels over n elements: kWidePerCu per CU at most,
 // `per` packs per thread per pass.
-inline unsigned wide_blocks(uint32_t n, int W, int cus, int per = 2) {
+inline unsigned wide_blocks(uint32_t n, int W, int cus, int per = 2, int per_cu = kWidePerCu) {
   const uint64_t packs = ((uint64_t)n + W - 1) / W;
   uint64_t g = (packs + (uint64_t)per * kWideThreads - 1) / ((uint64_t)per * kWideThreads);
   if (g < 1) g = 1;
-  if (g > (uint64_t)cus * kWidePerCu) g = (uint64_t)cus * kWidePerCu;
+  if (g > (uint64_t)cus * per_cu) g = (uint64_t)cus * per_cu;
   return (unsigned)g;
 }
 
