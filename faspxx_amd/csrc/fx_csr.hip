@@ -329,14 +329,22 @@ bool patterns_disabled() {
   return e && e[0] == '1';
 }
 
-// Sliced rows: 0 off (FASPXX_FX_NOSELL=1), 1 for matrices whose columns
-// take neither row patterns nor 16-bit deltas (default), 2 for every matrix
-// made from host arrays (FASPXX_FX_SELL=1).
+// Matrices whose columns take neither row patterns nor 16-bit deltas get
+// rows sorted by length (default; FASPXX_FX_NOSORT=1: not), else sliced
+// rows for the f64 kernels (FASPXX_FX_NOSELL=1: not).  FASPXX_FX_SELL=1
+// gives every matrix made from host arrays sliced rows, and no sorting.
+// sell_mode: 0 off, 1 where the columns are u32 and rows are not sorted,
+// 2 for every matrix.
 int sell_mode() {
   const char* e = std::getenv("FASPXX_FX_NOSELL");
   if (e && e[0] == '1') return 0;
   e = std::getenv("FASPXX_FX_SELL");
   return (e && e[0] == '1') ? 2 : 1;
+}
+
+bool sort_enabled() {
+  const char* e = std::getenv("FASPXX_FX_NOSORT");
+  return !(e && e[0] == '1');
 }
 
 }  // namespace
@@ -351,6 +359,40 @@ __global__ void k_sell_fill(const uint32_t* __restrict__ smap, const T* __restri
     const uint32_t m = smap[p];
     dst[p] = m == kSellNoRow ? (T)0 : src[m];
   }
+}
+
+// The sorted copy's values: dst[orp[s] + j] = src[rp[omap[s]] + j], one
+// thread per sorted row.
+template <class T>
+__global__ void k_sorted_fill(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ orp,
+                              const uint32_t* __restrict__ omap, const T* __restrict__ src,
+                              T* __restrict__ dst, uint32_t n) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t s = blockIdx.x * kThreads + threadIdx.x; s < n; s += stride) {
+    const uint32_t k0 = rp[omap[s]], d0 = orp[s], len = orp[s + 1] - d0;
+    for (uint32_t j = 0; j < len; ++j) dst[d0 + j] = src[k0 + j];
+  }
+}
+
+// Refresh the sorted copy's f64 values from va (fx_csr_make_f32 converts
+// them for the f32 copy).
+int sorted_fill(fx_csr A) {
+  if (!A->sorted || !A->has_values || A->nnz == 0) return 0;
+  if (!A->oval) FX_HIP(hipMalloc(&A->oval, sizeof(double) * A->nnz));
+  hipLaunchKernelGGL(k_sorted_fill<double>, dim3(stream_blocks(A->nrow)), dim3(kThreads), 0,
+                     A->ctx->stream, A->rp, A->orp, A->omap, A->va, A->oval, A->nrow);
+  FX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Values of a sorted matrix's copy from values src in the matrix's order
+// (same precision), into dst.
+int sorted_fill_f32(fx_csr A, const float* src, float* dst) {
+  if (!A->sorted || A->nnz == 0) return 0;
+  hipLaunchKernelGGL(k_sorted_fill<float>, dim3(stream_blocks(A->nrow)), dim3(kThreads), 0,
+                     A->ctx->stream, A->rp, A->orp, A->omap, src, dst, A->nrow);
+  FX_HIP(hipGetLastError());
+  return 0;
 }
 
 int sell_fill64(fx_csr A, const double* va, double* dst) {
@@ -437,6 +479,89 @@ int build_sell(fx_csr A, const uint32_t* hrp, const uint32_t* hci) {
   return 0;
 }
 
+// Build the sorted copy from the host row pointers and columns; its values
+// are filled on the device from A->va.  The rows are cut into kXcds
+// segments of about equal entry counts, and each segment's rows are sorted
+// by decreasing length (ties in row order).  Each segment's row blocks are
+// padded with empty blocks to the same count, so that xcd_block() runs
+// segment s on XCD s.  An XCD then reads and writes the vectors only in its
+// own rows' range: a global sort would scatter every XCD's writes over the
+// whole result.
+int build_sorted(fx_csr A, const uint32_t* hrp, const uint32_t* hci) {
+  const uint32_t n = A->nrow;
+  std::vector<uint32_t> seg(kXcds + 1, n);
+  seg[0] = 0;
+  for (uint32_t s = 1, r = 0; s < (uint32_t)kXcds; ++s) {
+    const uint64_t target = (uint64_t)A->nnz * s / kXcds;
+    while (r < n && hrp[r] < target) ++r;
+    seg[s] = r;
+  }
+  std::vector<uint32_t> omap(n);
+  for (uint32_t r = 0; r < n; ++r) omap[r] = r;
+  for (int s = 0; s < kXcds; ++s)
+    std::stable_sort(omap.begin() + seg[s], omap.begin() + seg[s + 1],
+                     [&](uint32_t a, uint32_t b) {
+                       return hrp[a + 1] - hrp[a] > hrp[b + 1] - hrp[b];
+                     });
+  std::vector<uint32_t> orp(n + 1), ocol(A->nnz ? A->nnz : 1);
+  orp[0] = 0;
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t r = omap[s], len = hrp[r + 1] - hrp[r];
+    std::copy(hci + hrp[r], hci + hrp[r + 1], ocol.begin() + orp[s]);
+    orp[s + 1] = orp[s] + len;
+  }
+  // per segment: blocks of the rows of more than kSeqRowLen entries, then of
+  // the shorter ones (never mixed, so that the short rows are summed one per
+  // thread, left to right)
+  std::vector<std::vector<uint32_t>> starts(kXcds);
+  size_t per = 0;
+  for (int s = 0; s < kXcds; ++s) {
+    const uint32_t s0 = seg[s], s1 = seg[s + 1];
+    uint32_t split = s0;
+    while (split < s1 && orp[split + 1] - orp[split] > kSeqRowLen) ++split;
+    uint32_t mx = 0;
+    for (const auto& range : {std::make_pair(s0, split), std::make_pair(split, s1)}) {
+      const std::vector<uint32_t> rb =
+          build_row_blocks(range.second - range.first, orp.data() + range.first, &mx);
+      for (size_t i = 0; i + 1 < rb.size(); ++i) starts[s].push_back(range.first + rb[i]);
+    }
+    per = std::max(per, starts[s].size());
+  }
+  if (per == 0) per = 1;
+  std::vector<uint32_t> rows;
+  rows.reserve(per * kXcds + 1);
+  for (int s = 0; s < kXcds; ++s) {
+    rows.insert(rows.end(), starts[s].begin(), starts[s].end());
+    rows.insert(rows.end(), per - starts[s].size(), seg[s + 1]);  // empty blocks
+  }
+  rows.push_back(n);
+  std::vector<uint32_t> blk(2 * rows.size());
+  for (size_t b = 0; b < rows.size(); ++b) {
+    blk[2 * b] = rows[b];
+    blk[2 * b + 1] = orp[rows[b]];
+  }
+  std::vector<uint16_t> rpb(n ? n : 1, 0);
+  for (size_t b = 0; b + 1 < rows.size(); ++b)
+    for (uint32_t r = rows[b]; r < rows[b + 1]; ++r) rpb[r] = (uint16_t)(orp[r] - orp[rows[b]]);
+  hipStream_t st = A->ctx->stream;
+  auto up = [&](auto** d, const auto& h) -> int {
+    using T = std::remove_reference_t<decltype(h[0])>;
+    FX_HIP(hipMalloc(d, sizeof(T) * h.size()));
+    FX_HIP(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+    return 0;
+  };
+  FX_TRY(up(&A->omap, omap));
+  FX_TRY(up(&A->orp, orp));
+  FX_TRY(up(&A->ocol, ocol));
+  FX_TRY(up(&A->oblk, blk));
+  FX_TRY(up(&A->orpb, rpb));
+  A->onblk = (uint32_t)rows.size() - 1;
+  A->sorted = true;
+  FX_TRY(sorted_fill(A));
+  FX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+  return 0;
+}
+
 int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
   // Row patterns for a matrix given on the host (FASPXX_FX_NOPAT=1: none;
   // FASPXX_FX_COL32=1 keeps u32 columns); they get their own row blocks.
@@ -510,7 +635,11 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
       FX_HIP(err);
     }
   }
-  if (!A->cd && (sell == 2 || (sell == 1 && !col32_env))) FX_TRY(build_sell(A, host_rp, host_ci));
+  if (host_ci && !A->cd && sell != 2 && !col32_env && sort_enabled() && A->nrow && A->nnz) {
+    FX_TRY(build_sorted(A, host_rp, host_ci));
+  } else if (!A->cd && (sell == 2 || (sell == 1 && !col32_env))) {
+    FX_TRY(build_sell(A, host_rp, host_ci));
+  }
   FX_HIP(hipStreamSynchronize(st));
   return 0;
 }
@@ -532,6 +661,8 @@ void csr_free(fx_csr A) {
   void* sell_arrays[] = {A->sblk, A->swin, A->soff, A->slane, A->scol,
                          A->smap, A->sval, A->lwin, A->lrow};
   for (void* q : sell_arrays) (void)hipFree(q);
+  void* sorted_arrays[] = {A->oblk, A->orpb, A->orp, A->omap, A->ocol, A->oval, A->oval32};
+  for (void* q : sorted_arrays) (void)hipFree(q);
   delete A;
 }
 
@@ -711,8 +842,11 @@ int fx_csr_index_bytes(fx_csr A, int* bytes) {
 int fx_csr_layout(fx_csr A, int* layout) {
   clear_error();
   FX_CHECK(A && layout, FX_ERROR_INPUT_PAR, "null argument");
-  *layout = A->pid ? FX_LAYOUT_PATTERN : A->cd ? FX_LAYOUT_D16 : A->sell ? FX_LAYOUT_SLICED
-                                                                       : FX_LAYOUT_U32;
+  *layout = A->pid      ? FX_LAYOUT_PATTERN
+            : A->cd     ? FX_LAYOUT_D16
+            : A->sorted ? FX_LAYOUT_SORTED
+            : A->sell   ? FX_LAYOUT_SLICED
+                        : FX_LAYOUT_U32;
   return 0;
 }
 
@@ -753,6 +887,12 @@ int fx_csr_make_f32(fx_csr A) {
   hipLaunchKernelGGL(k_f64_to_f32, dim3(stream_blocks(A->nnz)), dim3(kThreads), 0, A->ctx->stream,
                      A->va, A->va32, A->nnz);
   FX_HIP(hipGetLastError());
+  if (A->sorted) {  // the sorted copy's f32 values, converted from its f64 ones
+    if (!A->oval32) FX_HIP(hipMalloc(&A->oval32, sizeof(float) * A->nnz));
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(stream_blocks(A->nnz)), dim3(kThreads), 0,
+                       A->ctx->stream, A->oval, A->oval32, A->nnz);
+    FX_HIP(hipGetLastError());
+  }
   return 0;
 }
 
@@ -765,6 +905,7 @@ int fx_csr_set_values_dev(fx_csr A, const double* dev_values) {
                           A->ctx->stream));
   A->has_values = true;
   if (A->sell) FX_TRY(sell_fill64(A, A->va, A->sval));
+  if (A->sorted) FX_TRY(sorted_fill(A));
   if (A->va32) return fx_csr_make_f32(A);
   return 0;
 }

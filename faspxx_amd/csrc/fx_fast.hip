@@ -1562,6 +1562,7 @@ struct fx_ir_s {
   bool scaled = false;
   double* sc = nullptr;
   float* vs32 = nullptr;
+  float* vs32o = nullptr;  // vs32 in a sorted matrix's row order
   int* bad = nullptr;
   uint32_t* blk512 = nullptr;  // row-pattern matrices: As's own 512-row blocks
   uint16_t* rpb512 = nullptr;
@@ -1577,6 +1578,7 @@ struct fx_ir_s {
     (void)hipFree(part);
     (void)hipFree(sc);
     (void)hipFree(vs32);
+    (void)hipFree(vs32o);
     (void)hipFree(bad);
     (void)hipFree(blk512);
     (void)hipFree(rpb512);
@@ -1620,6 +1622,11 @@ int ir_scale(fx_ir ir) {
   FX_HIP(hipGetLastError());
   ir->As = *A;  // the same device arrays; only va32 differs, and nothing frees through As
   ir->As.va32 = ir->vs32;
+  if (A->sorted) {  // the scaled values in the sorted copy's order too
+    FX_HIP(hipMalloc(&ir->vs32o, sizeof(float) * (A->nnz ? A->nnz : 1)));
+    FX_TRY(sorted_fill_f32(A, ir->vs32, ir->vs32o));
+    ir->As.oval32 = ir->vs32o;
+  }
   if (A->pid) {
     // the f32 inner SpMV of a row-pattern matrix runs on 512-row blocks:
     // IR 10.6 against 11.1 ms on the 128^3 stencil (half the partial sums

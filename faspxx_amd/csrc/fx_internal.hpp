@@ -158,6 +158,23 @@ struct fx_csr_s {
   double* sval = nullptr;     // sslots: f64 values (padding: 0)
   uint32_t* lwin = nullptr;   // nwin + 1: first long row of each window in lrow
   uint32_t* lrow = nullptr;   // rows of more than kSellMax entries
+  // Rows sorted by length, for matrices made from host arrays whose columns
+  // take neither row patterns nor 16-bit deltas (irregular matrices), in
+  // both precisions: a copy of the matrix with its rows in order of
+  // decreasing length (ties in row order), cut into row blocks as above
+  // (oblk / onblk / orpb / ocol / oval / oval32 in the formats of blk / nblk
+  // / rpb / ci / va / va32).  Sorted row s is row omap[s]; each row keeps
+  // its entries in their order, so every row is summed as before and only
+  // the assignment of rows to workgroups changes.
+  bool sorted = false;
+  uint32_t onblk = 0;
+  uint32_t* oblk = nullptr;
+  uint16_t* orpb = nullptr;
+  uint32_t* orp = nullptr;   // n + 1: row pointers of the sorted copy
+  uint32_t* omap = nullptr;  // n: the matrix row of each sorted row
+  uint32_t* ocol = nullptr;
+  double* oval = nullptr;
+  float* oval32 = nullptr;
 };
 
 struct fx_pc_s {
@@ -211,6 +228,10 @@ int csr_spmv_dispatch(fx_ctx ctx, fx_csr A, const void* x, void* y, int dtype,
 int ensure_dinv32(fx_pc pc);
 // Slice values of a sliced-row matrix from CSR-ordered values (fx_csr.hip).
 int sell_fill64(fx_csr A, const double* va, double* dst);
+// The sorted copy's values from the matrix's (f64: A->va into A->oval;
+// f32: src, in the matrix's entry order, into dst).
+int sorted_fill(fx_csr A);
+int sorted_fill_f32(fx_csr A, const float* src, float* dst);
 
 // Host-side launch geometry for streaming kernels over n elements.
 inline unsigned stream_blocks(uint32_t n, uint32_t per_thread = 4) {
@@ -233,6 +254,7 @@ inline unsigned spmv_grid(uint32_t nblk, int /*cus*/) {
 // has them, else the row blocks.
 template <class MT>
 inline uint32_t spmv_units(const fx_csr_s* A) {
+  if (A->sorted) return A->onblk;
   return (sizeof(MT) == 8 && A->sell) ? A->nwin : A->nblk;
 }
 template <class MT>

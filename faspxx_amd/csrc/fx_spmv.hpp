@@ -56,6 +56,9 @@ struct SpmvIn {
   const MT* __restrict__ sval;  // null: structure-only matrix
   const uint32_t* __restrict__ lwin;
   const uint32_t* __restrict__ lrow;
+  // rows sorted by length (u32 columns): the matrix row of each block row,
+  // or null when block rows are matrix rows
+  const uint32_t* __restrict__ rmap;
 };
 
 // Column formats of the staged entries: u32 columns, 16-bit deltas to the
@@ -71,19 +74,31 @@ inline SpmvIn<MT, VT> spmv_in(fx_csr A);
 // on the row blocks (the f32 kernels get no slice arrays).
 template <>
 inline SpmvIn<double, double> spmv_in<double, double>(fx_csr A) {
+  if (A->sorted)
+    return {A->orp,   A->orpb, A->ocol, nullptr, A->has_values ? A->oval : nullptr,
+            A->oblk,  A->onblk, A->nnz, nullptr, nullptr,
+            0,        0,
+            nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, nullptr, A->omap};
   return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va : nullptr,
           A->sell ? A->sblk : A->blk,  A->sell ? A->nwin : A->nblk, A->nnz, A->pid,   A->pdict,
           A->plen, A->npat * A->plen,
           A->swin, A->soff, A->slane, A->scol, A->has_values ? A->sval : nullptr,
-          A->lwin, A->lrow};
+          A->lwin, A->lrow, nullptr};
 }
 template <>
 inline SpmvIn<float, float> spmv_in<float, float>(fx_csr A) {
+  if (A->sorted)
+    return {A->orp,   A->orpb, A->ocol, nullptr, A->has_values ? A->oval32 : nullptr,
+            A->oblk,  A->onblk, A->nnz, nullptr, nullptr,
+            0,        0,
+            nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, nullptr, A->omap};
   return {A->rp,   A->rpb,  A->ci,  A->cd,    A->has_values ? A->va32 : nullptr,
           A->blk,  A->nblk, A->nnz, A->pid,   A->pdict,
           A->plen, A->npat * A->plen,
           nullptr, nullptr, nullptr, nullptr, nullptr,
-          nullptr, nullptr};
+          nullptr, nullptr, nullptr};
 }
 
 // Threads per row for a block of nr rows holding nz entries: one thread per
@@ -570,6 +585,11 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
     const uint32_t r0 = in.blk[2 * bid], k0 = in.blk[2 * bid + 1];
     const uint32_t r1 = in.blk[2 * bid + 2], k1 = in.blk[2 * bid + 3];
     const uint32_t nr = r1 - r0, nz = k1 - k0;
+    // the matrix row of block row r (rows sorted by length: u32 columns only)
+    auto orow = [&](uint32_t r) -> uint32_t {
+      if constexpr (CF == CF_U32) return in.rmap ? in.rmap[r] : r;
+      else return r;
+    };
     if (!is_pat(CF) && nz > kBlockNnz) {  // (row-pattern rows are short)
       // a single long row: strided partial sums (4 strides' loads in
       // flight at a time, summed in stride order), then a workgroup reduction
@@ -595,8 +615,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
       double t[1] = {s};
       block_reduce<1>(t);
       if (tid == 0) {
-        const double v = (MODE == ROW_AX) ? t[0] : (double)bf(r0) - t[0];
-        epi.row(r0, v, acc);
+        const uint32_t orr = orow(r0);
+        const double v = (MODE == ROW_AX) ? t[0] : (double)bf(orr) - t[0];
+        epi.row(orr, v, acc);
       }
     } else if constexpr (is_pat(CF)) {
       in.va ? pat_block<CF, true, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc)
@@ -650,8 +671,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
               double tt[1] = {t};
               block_reduce<1>(tt);
               if (tid == 0) {
-                const double v = (MODE == ROW_AX) ? tt[0] : (double)bf(r0 + rr) - tt[0];
-                epi.row(r0 + rr, v, acc);
+                const uint32_t orr = orow(r0 + rr);
+                const double v = (MODE == ROW_AX) ? tt[0] : (double)bf(orr) - tt[0];
+                epi.row(orr, v, acc);
               }
             }
           }
@@ -659,7 +681,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
         for (uint32_t row = tid; row < nr; row += kThreads) {
           const uint32_t s = lds.rp[row], e = lds.rp[row + 1];
           if (nlong && e - s > kLongRow) continue;  // summed above
-          double t = (MODE == ROW_AX) ? 0.0 : (double)bf(r0 + row);
+          const uint32_t orr = orow(r0 + row);
+          double t = (MODE == ROW_AX) ? 0.0 : (double)bf(orr);
           // chunks of 8: the chunk's LDS reads, then its gathers, are all
           // issued before its fma chain (indices past the row's end are
           // clamped to its last entry and their terms dropped)
@@ -685,7 +708,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
               }
             }
           }
-          epi.row(r0 + row, t, acc);
+          epi.row(orr, t, acc);
         }
       } else {
         const int grp = tid / tpr, lane = tid % tpr, ngrp = kThreads / tpr;
@@ -710,8 +733,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
           }
           t = group_sum(t, tpr);
           if (lane == 0) {
-            const double v = (MODE == ROW_AX) ? t : (double)bf(r0 + rr) - t;
-            epi.row(r0 + rr, v, acc);
+            const uint32_t orr = orow(r0 + rr);
+            const double v = (MODE == ROW_AX) ? t : (double)bf(orr) - t;
+            epi.row(orr, v, acc);
           }
         }
       }

@@ -450,10 +450,11 @@ class Csr:
 
     @property
     def layout(self) -> str:
-        """How the SpMV reads the matrix: 'u32', 'd16', 'pattern' or 'sliced'."""
+        """How the SpMV reads the matrix: 'u32', 'd16', 'pattern', 'sliced' or
+        'sorted'."""
         b = C.c_int()
         _chk(load().fx_csr_layout(self.h, C.byref(b)))
-        return ("u32", "d16", "pattern", "sliced")[b.value]
+        return ("u32", "d16", "pattern", "sliced", "sorted")[b.value]
 
     @property
     def sliced(self) -> bool:

@@ -191,18 +191,25 @@ int fx_csr_info(fx_csr A, uint32_t* nrow, uint32_t* mcol, uint32_t* nnz);
 int fx_csr_index_bytes(fx_csr A, int* bytes);
 /* How the SpMV kernels read the matrix: FX_LAYOUT_U32 (row blocks, u32
  * columns), FX_LAYOUT_D16 (row blocks, 16-bit column deltas),
- * FX_LAYOUT_PATTERN (row patterns), FX_LAYOUT_SLICED (for the f64 kernels,
- * windows of 256 rows whose rows of at most 32 entries are sorted by length
- * into 64-row slices stored column-major, longer rows a wavefront or the
- * workgroup each; the f32 kernels keep u32 row blocks: the default for a
- * matrix made from host arrays whose columns suit neither row patterns nor
- * 16-bit deltas; FASPXX_FX_NOSELL=1 at creation turns it off,
- * FASPXX_FX_SELL=1 forces it).  Rows of at most 32 entries give the same
- * bits in every layout.  No reference counterpart. */
+ * FX_LAYOUT_PATTERN (row patterns), FX_LAYOUT_SORTED (u32 row blocks over a
+ * copy of the matrix whose rows are in order of decreasing length, each row
+ * written to its own slot: the default for a matrix made from host arrays
+ * whose columns suit neither row patterns nor 16-bit deltas;
+ * FASPXX_FX_NOSORT=1 at creation turns it off), FX_LAYOUT_SLICED (for the
+ * f64 kernels, windows of 256 rows whose rows of at most 32 entries are
+ * sorted by length into 64-row slices stored column-major, longer rows a
+ * wavefront or the workgroup each; the f32 kernels keep u32 row blocks: used
+ * with FASPXX_FX_NOSORT=1, or for every matrix made from host arrays with
+ * FASPXX_FX_SELL=1; FASPXX_FX_NOSELL=1 turns it off).  A row summed by one
+ * thread, left to right (in the sliced layout every row of at most 32
+ * entries; in row blocks the rows of blocks averaging at most 32 entries per
+ * row), gives the same bits in every layout; rows summed by lane groups
+ * agree to rounding.  No reference counterpart. */
 #define FX_LAYOUT_U32 0
 #define FX_LAYOUT_D16 1
 #define FX_LAYOUT_PATTERN 2
 #define FX_LAYOUT_SLICED 3
+#define FX_LAYOUT_SORTED 4
 int fx_csr_layout(fx_csr A, int* layout);
 /* Copy the device arrays back (any pointer may be NULL). */
 int fx_csr_download(fx_csr A, uint32_t* rowPtr, uint32_t* colInd, double* values,

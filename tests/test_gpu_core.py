@@ -139,13 +139,15 @@ def test_column_delta_format(ctx, problems, monkeypatch, cid, scale, nbytes):
     16-bit reach (at n = 16384 every column is within 32K of every row)."""
     pr = problems(cid, scale)
     monkeypatch.setenv("FASPXX_FX_NOPAT", "1")  # row patterns off: deltas where they fit
-    monkeypatch.setenv("FASPXX_FX_NOSELL", "1")  # no sliced rows where deltas do not fit
+    monkeypatch.setenv("FASPXX_FX_NOSELL", "1")  # no sliced or sorted rows where deltas do not fit
+    monkeypatch.setenv("FASPXX_FX_NOSORT", "1")
     A = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.setenv("FASPXX_FX_COL32", "1")
     A32 = fx.Csr.from_problem(ctx, pr.A)
     monkeypatch.delenv("FASPXX_FX_COL32")
     monkeypatch.delenv("FASPXX_FX_NOPAT")
     monkeypatch.delenv("FASPXX_FX_NOSELL")
+    monkeypatch.delenv("FASPXX_FX_NOSORT")
     assert A.index_bytes == nbytes and A32.index_bytes == 4
     x = dev(ctx, np.random.default_rng(7).uniform(-1, 1, pr.A.nrow))
     w, w32 = fx.Vec(ctx, pr.A.nrow), fx.Vec(ctx, pr.A.nrow)
@@ -433,9 +435,13 @@ def test_sliced_format(ctx, oracle, problems, monkeypatch, cid, scale):
     S, U = _sliced_pair(ctx, pr.A, monkeypatch)
     assert S.index_bytes == 4 and S.sliced and not U.sliced
     _check_sliced(ctx, oracle, pr.A, S, U, cid)
-    if cid == 4 and scale == 200000:  # the default form there is sliced
+    if cid == 4 and scale == 200000:  # the default form there sorts the rows by length
         D = fx.Csr.from_problem(ctx, pr.A)
-        assert D.sliced
+        assert D.layout == "sorted"
+        monkeypatch.setenv("FASPXX_FX_NOSORT", "1")
+        D2 = fx.Csr.from_problem(ctx, pr.A)
+        monkeypatch.delenv("FASPXX_FX_NOSORT")
+        assert D2.sliced
 
 
 def test_sliced_rectangular_empty_and_long_rows(ctx, oracle, monkeypatch):
@@ -468,6 +474,98 @@ def test_sliced_rectangular_empty_and_long_rows(ctx, oracle, monkeypatch):
             short = lens <= 32
             assert np.array_equal(w.numpy()[short], ref[short])
             assert rel_err(w.numpy()[~short], ref[~short]) < 1e-12
+
+
+def _check_sorted(ctx, oracle, Acsr, D, seed):
+    """Apply (f64, f32) and Residual of a matrix with rows sorted by length:
+    every row is summed by the u32 row-block code, and the blocks keep rows
+    of at most 32 entries apart from longer ones, so those rows (one thread
+    each, left to right) equal the oracle bit for bit and the rest agree to
+    1e-12 relative; the x = 0 shortcut gives b."""
+    n, m = Acsr.nrow, Acsr.mcol
+    M = Matrix.of(Acsr)
+    lens = np.diff(Acsr.row_ptr.astype(np.int64))
+    short = lens <= 32
+    xh = np.random.default_rng(seed).uniform(-1, 1, m)
+    w = fx.Vec(ctx, n)
+    D.Apply(dev(ctx, xh), w)
+    ref = oracle.spmv(M, xh)
+    assert np.array_equal(w.numpy()[short], ref[short])
+    assert rel_err(w.numpy(), ref) < 1e-12
+    if n == m:
+        b = Acsr_b(Acsr, seed)
+        r = fx.Vec(ctx, n)
+        D.Residual(dev(ctx, b), dev(ctx, xh), r)
+        rref = oracle.residual(M, b, xh)
+        assert np.array_equal(r.numpy()[short], rref[short])
+        assert rel_err(r.numpy(), rref) < 1e-12
+        D.Residual(dev(ctx, b), dev(ctx, np.zeros(n)), r)
+        assert np.array_equal(r.numpy(), b)
+    D.make_f32()
+    a = fx.Vec(ctx, n, fx.F32)
+    D.Apply(dev(ctx, xh.astype(np.float32), fx.F32), a)
+    assert rel_err(a.numpy(), ref) < 1e-5
+
+
+def test_sorted_rows_powerlaw(ctx, oracle, problems):
+    """The power-law system past the 16-bit reach takes rows sorted by
+    length by default."""
+    pr = problems(4, 200000)
+    D = fx.Csr.from_problem(ctx, pr.A)
+    assert D.layout == "sorted" and D.index_bytes == 4
+    _check_sorted(ctx, oracle, pr.A, D, 4)
+
+
+def test_sorted_rows_rectangular_empty_and_long(ctx, oracle):
+    """Rectangular matrices (n > m and n < m) with empty rows, rows longer
+    than a block's 1792 entries and many short rows, with values and
+    structure-only (columns too scattered for 16-bit deltas)."""
+    rng = np.random.default_rng(11)
+    for n, m in ((80000, 50000), (6000, 90000)):  # columns past the 16-bit reach
+        lens = rng.integers(0, 12, n)
+        lens[rng.random(n) < 0.1] = 0
+        lens[rng.integers(0, n, 30)] = rng.integers(33, 300, 30)
+        lens[rng.integers(0, n, 4)] = rng.integers(1800, min(m, 4000), 4)
+        lens = np.minimum(lens, m)
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+        ci = np.concatenate([np.sort(rng.choice(m, k, replace=False)) for k in lens]).astype(
+            np.uint32)
+        va = rng.uniform(-1, 1, int(rp[-1]))
+        for values in (va, None):
+            D = fx.Csr(ctx, n, m, rp, ci, values, None)
+            assert D.layout == "sorted"
+            xh = rng.uniform(-1, 1, m)
+            w = fx.Vec(ctx, n)
+            D.Apply(dev(ctx, xh), w)
+            M = Matrix(n, m, rp, ci, values, oracle.form_diagptr(n, rp, ci))
+            ref = oracle.spmv(M, xh)
+            short = lens <= 32
+            assert np.array_equal(w.numpy()[short], ref[short])
+            assert rel_err(w.numpy(), ref) < 1e-12
+
+
+def test_sorted_rows_values_refresh(ctx, oracle, problems):
+    """New values (set on the device, a diagonal shift) reach the sorted
+    copy, in f64 and f32."""
+    pr = problems(4, 200000)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    assert A.layout == "sorted"
+    A.make_f32()
+    M = Matrix.of(pr.A)
+    xh = np.random.default_rng(3).uniform(-1, 1, pr.A.nrow)
+    short = np.diff(pr.A.row_ptr.astype(np.int64)) <= 32
+    nv = dev(ctx, pr.A.values * -1.5)
+    A.set_values_dev(nv.data_ptr)
+    A.shift_diag(0.25)
+    want_vals = oracle.mat_shift(Matrix(M.nrow, M.mcol, M.rp, M.ci, pr.A.values * -1.5, M.dp), 0.25)
+    w = fx.Vec(ctx, pr.A.nrow)
+    A.Apply(dev(ctx, xh), w)
+    ref = oracle.spmv(Matrix(M.nrow, M.mcol, M.rp, M.ci, want_vals, M.dp), xh)
+    assert np.array_equal(w.numpy()[short], ref[short])
+    assert rel_err(w.numpy(), ref) < 1e-12
+    a = fx.Vec(ctx, pr.A.nrow, fx.F32)
+    A.Apply(dev(ctx, xh.astype(np.float32), fx.F32), a)
+    assert rel_err(a.numpy(), ref) < 1e-5
 
 
 def test_sliced_values_refresh(ctx, oracle, problems, monkeypatch):
