@@ -685,16 +685,25 @@ struct BiRun {
     const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
     uint64_t host_steps = 0;
     for (;;) {
+      ReplayPace pace;
+      pace.hint = host_steps == 0 ? W.last_iters : 0;
+      int inflight = pace.initial();  // slot cur finishes next
       FX_TRY(launch_chunk(0));
-      FX_TRY(launch_chunk(1));
-      uint64_t chunks = 2;
+      if (inflight == 2) FX_TRY(launch_chunk(1));
+      uint64_t chunks = inflight;
       int cur = 0;
       for (;;) {
         FX_HIP(hipEventSynchronize(W.ev[cur]));
-        if (hslot(cur)->status != ST_RUN) break;
+        --inflight;
+        const BiState* hs = hslot(cur);
+        if (hs->status != ST_RUN) break;
         FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "BiCGStab device loop did not stop");
-        FX_TRY(launch_chunk(cur));
-        ++chunks;
+        const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
+        if (pace.more(hs->resAbs, target, hs->numIter) || inflight == 0) {
+          FX_TRY(launch_chunk(cur));
+          ++chunks;
+          ++inflight;
+        }
         cur ^= 1;
       }
       FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
@@ -705,6 +714,7 @@ struct BiRun {
       FX_TRY(write_state(s));
     }
     res->numIter = s.numIter;
+    W.last_iters = host_steps == 1 ? s.numIter : 0;  // a prediction only for plain runs
     if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
       res->norm2 = s.resAbs;
       res->normInf = o.norminf(vr_);

@@ -1236,17 +1236,25 @@ struct CgRun {
     const uint64_t max_chunks = (uint64_t)PP.maxIter / kChunk + 4;
     uint64_t host_steps = 0;
     for (;;) {
+      ReplayPace pace;
+      pace.hint = host_steps == 0 ? W.last_iters : 0;
+      int inflight = pace.initial();  // slot cur finishes next
       FX_TRY(launch_chunk(0));
-      FX_TRY(launch_chunk(1));
-      uint64_t chunks = 2;
+      if (inflight == 2) FX_TRY(launch_chunk(1));
+      uint64_t chunks = inflight;
       int cur = 0;
       for (;;) {
         FX_HIP(hipEventSynchronize(W.ev[cur]));
+        --inflight;
         const CgState* hs = (const CgState*)W.hstate + cur;
         if (hs->status != ST_RUN) break;
         FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "CG device loop did not stop");
-        FX_TRY(launch_chunk(cur));
-        ++chunks;
+        const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
+        if (pace.more(hs->resAbs, target, hs->numIter) || inflight == 0) {
+          FX_TRY(launch_chunk(cur));
+          ++chunks;
+          ++inflight;
+        }
         cur ^= 1;
       }
       FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
@@ -1259,6 +1267,7 @@ struct CgRun {
       FX_TRY(write_state(s));
     }
     res->numIter = s.numIter;
+    W.last_iters = host_steps == 1 ? s.numIter : 0;  // a prediction only for plain runs
     if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
       res->norm2 = s.resAbs;
       res->normInf = o.norminf(vr);
@@ -1443,23 +1452,31 @@ struct IrInner {
   // Replays until the device loop stops; out = the final state.  A replay
   // that finds the loop running advances it by kChunk iterations, so it
   // stops within maxIter / kChunk + 1 replays.
-  int run(uint32_t maxIter, IrState& out) {
+  // hint: the iterations of the previous inner solve (0: none)
+  int run(uint32_t maxIter, IrState& out, uint32_t hint) {
     FX_TRY(build_graph());
     const uint64_t max_chunks = (uint64_t)maxIter / kChunk + 4;
+    ReplayPace pace;
+    pace.hint = hint;
+    int inflight = pace.initial();  // slot cur finishes next
     FX_TRY(launch_chunk(0));
-    FX_TRY(launch_chunk(1));
-    uint64_t chunks = 2;
+    if (inflight == 2) FX_TRY(launch_chunk(1));
+    uint64_t chunks = inflight;
     int cur = 0;
     for (;;) {
       FX_HIP(hipEventSynchronize(W.ev[cur]));
+      --inflight;
       const IrState* hs = hslot(cur);
       if (hs->status != ST_RUN) {
         out = *hs;
         return 0;
       }
       FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "IR inner loop did not stop");
-      FX_TRY(launch_chunk(cur));
-      ++chunks;
+      if (pace.more(hs->gamma, hs->thr, hs->numIter) || inflight == 0) {  // squared norms
+        FX_TRY(launch_chunk(cur));
+        ++chunks;
+        ++inflight;
+      }
       cur ^= 1;
     }
   }
@@ -1755,8 +1772,9 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         // S A S y = S r in f32 from y = 0; x += S y
         IrInner in{*ir->W, &ir->As, st};
         IrState fs{};
-        rc = in.run(ir->inner.maxIter, fs);
+        rc = in.run(ir->inner.maxIter, fs, ir->W->last_iters);
         if (rc) break;
+        ir->W->last_iters = fs.reason == IR_CONVERGED ? fs.numIter : 0;
         inner_total += fs.numIter;
         if (fs.numIter > 0) {
           hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,

@@ -2,6 +2,8 @@
 // (fx_fast.hip: CG, Jacobi, mixed precision; fx_bicgstab.hip: BiCGStab).
 #pragma once
 
+#include <cmath>
+
 #include "fx_solver.hpp"
 #include "fx_spmv.hpp"
 
@@ -68,6 +70,37 @@ inline unsigned wide_blocks(uint32_t n, int W, int cus, int per = 2, int per_cu 
 
 void gmres_ws_free(void* p);
 
+// Replay loops keep two replays in flight, so that the GPU never waits for
+// the host between them; but the one behind the replay that ends the loop
+// runs only skipped kernels (16 iterations' worth: about 56 us for the
+// FP64 CG).  After each finished replay this predicts, from how fast a
+// measure that the loop drives below a target (a residual norm) fell over
+// the last replay, whether the replay still in flight ends the loop, in
+// which case no other is queued yet.
+// When the same loop ran before (a solver's earlier solve, the refinement's
+// earlier inner solve), its iteration count serves as the prediction
+// instead, also for the first replays.  A wrong prediction costs one wait
+// for the host between replays.
+struct ReplayPace {
+  double prev = -1.0;
+  uint32_t hint = 0;  // iterations the loop took last time; 0: unknown
+  // replays to queue at the start
+  int initial() const { return (hint && hint <= (uint32_t)kChunk) ? 1 : 2; }
+  // m: the measure after the replay that just finished, target: the value
+  // that stops the loop, done: iterations run so far.  True when another
+  // replay should be queued now.
+  bool more(double m, double target, uint32_t done) {
+    const double p = prev;
+    prev = m;
+    if (hint) return (int64_t)hint - (int64_t)done > (int64_t)kChunk;
+    if (!(p > 0.0) || !(m > 0.0) || !(target > 0.0) || !(target < m)) return true;
+    const double f = m / p;  // per replay
+    if (!(f < 1.0)) return true;
+    // replays still needed after this one; the one in flight is one of them
+    return std::log(target / m) / std::log(f) > 0.9;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Workspace
 // ---------------------------------------------------------------------------
@@ -94,6 +127,7 @@ struct FastWs {
   void* opy = nullptr;         // callback operator: apply's output
   void* opx = nullptr;         // callback operator: an operand formed before apply
   size_t part2_off = 0;        // second partials region (doubles)
+  uint32_t last_iters = 0;     // iterations of the last solve on this workspace
   hipGraphExec_t chunk[2] = {nullptr, nullptr};  // replay graphs, one per host state slot
   hipEvent_t ev[2] = {nullptr, nullptr};
 
