@@ -699,7 +699,8 @@ struct BiRun {
         if (hs->status != ST_RUN) break;
         FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "BiCGStab device loop did not stop");
         const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
-        if (pace.more(hs->resAbs, target, hs->numIter) || inflight == 0) {
+        if (pace.more(hs->resAbs, target, hs->numIter, PP.minIter == 0 ? hs->denAbs : 0.0) ||
+            inflight == 0) {
           FX_TRY(launch_chunk(cur));
           ++chunks;
           ++inflight;

@@ -1250,7 +1250,8 @@ struct CgRun {
         if (hs->status != ST_RUN) break;
         FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "CG device loop did not stop");
         const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
-        if (pace.more(hs->resAbs, target, hs->numIter) || inflight == 0) {
+        if (pace.more(hs->resAbs, target, hs->numIter, PP.minIter == 0 ? hs->denAbs : 0.0) ||
+            inflight == 0) {
           FX_TRY(launch_chunk(cur));
           ++chunks;
           ++inflight;
@@ -1472,7 +1473,7 @@ struct IrInner {
         return 0;
       }
       FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "IR inner loop did not stop");
-      if (pace.more(hs->gamma, hs->thr, hs->numIter) || inflight == 0) {  // squared norms
+      if (pace.more(hs->gamma, hs->thr, hs->numIter, hs->gamma0) || inflight == 0) {  // squared
         FX_TRY(launch_chunk(cur));
         ++chunks;
         ++inflight;
@@ -1580,6 +1581,7 @@ struct fx_ir_s {
   double* sc = nullptr;
   float* vs32 = nullptr;
   float* vs32o = nullptr;  // vs32 in a sorted matrix's row order
+  std::vector<uint32_t> inner_counts;  // each outer step's inner iterations, last solve
   int* bad = nullptr;
   uint32_t* blk512 = nullptr;  // row-pattern matrices: As's own 512-row blocks
   uint16_t* rpb512 = nullptr;
@@ -1772,9 +1774,13 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         // S A S y = S r in f32 from y = 0; x += S y
         IrInner in{*ir->W, &ir->As, st};
         IrState fs{};
-        rc = in.run(ir->inner.maxIter, fs, ir->W->last_iters);
+        // the prediction: this outer step's inner count in the last solve
+        const size_t step = outer_it - 1;
+        const uint32_t hint = step < ir->inner_counts.size() ? ir->inner_counts[step] : 0u;
+        rc = in.run(ir->inner.maxIter, fs, hint);
         if (rc) break;
-        ir->W->last_iters = fs.reason == IR_CONVERGED ? fs.numIter : 0;
+        if (ir->inner_counts.size() <= step) ir->inner_counts.resize(step + 1, 0u);
+        ir->inner_counts[step] = fs.reason == IR_CONVERGED ? fs.numIter : 0;
         inner_total += fs.numIter;
         if (fs.numIter > 0) {
           hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,

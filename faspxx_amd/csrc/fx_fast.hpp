@@ -87,10 +87,11 @@ struct ReplayPace {
   // replays to queue at the start
   int initial() const { return (hint && hint <= (uint32_t)kChunk) ? 1 : 2; }
   // m: the measure after the replay that just finished, target: the value
-  // that stops the loop, done: iterations run so far.  True when another
-  // replay should be queued now.
-  bool more(double m, double target, uint32_t done) {
-    const double p = prev;
+  // that stops the loop, done: iterations run so far, init: the measure
+  // before the loop's first replay (0: unknown).  True when another replay
+  // should be queued now.
+  bool more(double m, double target, uint32_t done, double init) {
+    const double p = prev > 0.0 ? prev : init;
     prev = m;
     if (hint) return (int64_t)hint - (int64_t)done > (int64_t)kChunk;
     if (!(p > 0.0) || !(m > 0.0) || !(target > 0.0) || !(target < m)) return true;
