@@ -687,25 +687,26 @@ struct BiRun {
     for (;;) {
       ReplayPace pace;
       pace.hint = host_steps == 0 ? W.last_iters : 0;
-      int inflight = pace.initial();  // slot cur finishes next
-      FX_TRY(launch_chunk(0));
-      if (inflight == 2) FX_TRY(launch_chunk(1));
-      uint64_t chunks = inflight;
-      int cur = 0;
+      SlotQueue sq;
+      for (int i = pace.initial(); i > 0; --i) {
+        FX_TRY(launch_chunk(sq.free_slot()));
+        sq.push(sq.free_slot());
+      }
+      uint64_t chunks = sq.n;
       for (;;) {
+        const int cur = sq.front();
         FX_HIP(hipEventSynchronize(W.ev[cur]));
-        --inflight;
+        sq.pop();
         const BiState* hs = hslot(cur);
         if (hs->status != ST_RUN) break;
         FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "BiCGStab device loop did not stop");
         const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
         if (pace.more(hs->resAbs, target, hs->numIter, PP.minIter == 0 ? hs->denAbs : 0.0) ||
-            inflight == 0) {
-          FX_TRY(launch_chunk(cur));
+            sq.n == 0) {
+          FX_TRY(launch_chunk(sq.free_slot()));
+          sq.push(sq.free_slot());
           ++chunks;
-          ++inflight;
         }
-        cur ^= 1;
       }
       FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
                "BiCGStab device loop did not stop");

@@ -102,6 +102,31 @@ struct ReplayPace {
   }
 };
 
+// The host slots of the replays in flight, oldest first (at most two).  A
+// replay is queued on the slot that no replay in flight uses, and the host
+// waits for the oldest one, so that it never reads a slot that no replay of
+// the current loop has written.
+struct SlotQueue {
+  int q[2] = {0, 0};
+  int n = 0;
+  int free_slot() const { return n == 0 ? 0 : q[n - 1] ^ 1; }
+  void push(int s) { q[n++] = s; }
+  int front() const { return q[0]; }
+  void pop() {
+    q[0] = q[1];
+    --n;
+  }
+};
+
+// Iterations of the next replay: kChunk, or for the replay that is predicted
+// to end the loop (need: the iterations the loop is predicted to launch, 0:
+// no prediction; queued: those queued so far) the even count that covers the
+// rest.  Even, so that every later replay starts on the same state parity.
+inline int replay_len(uint64_t need, uint64_t queued) {
+  if (need == 0 || queued >= need || need - queued >= (uint64_t)kChunk) return kChunk;
+  return (int)((need - queued + 1) & ~(uint64_t)1);
+}
+
 // ---------------------------------------------------------------------------
 // Workspace
 // ---------------------------------------------------------------------------
@@ -130,6 +155,9 @@ struct FastWs {
   size_t part2_off = 0;        // second partials region (doubles)
   uint32_t last_iters = 0;     // iterations of the last solve on this workspace
   hipGraphExec_t chunk[2] = {nullptr, nullptr};  // replay graphs, one per host state slot
+  // shorter replays (2, 4, .. kChunk - 2 iterations) for the replay that is
+  // predicted to end a solve, per host state slot; made on first use
+  hipGraphExec_t tail[2][kChunk / 2] = {};
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   ~FastWs();
@@ -144,6 +172,9 @@ inline FastWs::~FastWs() {
     if (hstate) (void)hipHostFree(hstate);
     for (auto& c : chunk)
       if (c) (void)hipGraphExecDestroy(c);
+    for (auto& slot : tail)
+      for (auto& c : slot)
+        if (c) (void)hipGraphExecDestroy(c);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
   }

@@ -535,3 +535,64 @@ def test_gmres_safeguard_fused(ctx, oracle, problems, kind, okind, savIter):
             assert np.array_equal(x, x0) and np.array_equal(o["x"], x0)
         else:
             assert rel_err(x, o["x"]) < 1e-6
+
+
+def _sine_mode(m):
+    """An eigenvector of the m^3 7-point Laplacian (Dirichlet-zero boundary):
+    with it as the right-hand side, Jacobi-PCG and the refinement's scaled
+    CG stop after one iteration."""
+    t = np.sin(np.pi * np.arange(1, m + 1) / (m + 1))
+    return np.einsum("k,j,i->kji", t, t, t).reshape(-1).copy()
+
+
+@pytest.mark.parametrize("kind", [fx.CG, fx.BICGSTAB])
+def test_solve_after_short_solve(ctx, problems, kind):
+    """A solve on a set-up solver whose previous solve stopped within one
+    replay: the replay loop is then predicted to stop within its first
+    replay, finds it running, and must go on waiting for the replays of
+    this solve only (not a host slot that the previous solve left holding
+    its final state).  Same count and bit-identical x as a fresh solver."""
+    m = 32
+    pr = problems(2, m)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "jacobi", A, 1.0, 1)
+    s = fx.Solver(ctx, kind)
+    s.Setup(A, P, fx.params(**PRM))
+    x = dev(ctx, np.zeros(pr.A.nrow))
+    short = s.Solve(dev(ctx, pr.A.matvec(_sine_mode(m))), x)
+    assert short.retcode == 0 and 1 <= short.numIter <= 2, short.numIter
+    x = dev(ctx, np.zeros(pr.A.nrow))
+    res = s.Solve(dev(ctx, pr.b), x)
+    assert s.last_path == "fused"
+    ref, xref, _ = gpu_solve(ctx, pr, kind, "jacobi", **PRM)
+    assert res.retcode == ref.retcode == 0
+    assert res.numIter == ref.numIter > 40, (res.numIter, ref.numIter)
+    assert np.array_equal(x.numpy(), xref)
+
+
+def test_ir_solve_after_short_solve(ctx, problems):
+    """The refinement's inner solves are predicted from the same outer step
+    of the previous solve; after a solve whose first inner solve took one
+    iteration (the right-hand side is an eigenvector), a solve whose first
+    inner solve needs many runs it to the end: same outer and inner counts
+    and bit-identical x as a fresh refinement object."""
+    m = 32
+    pr = problems(2, m)
+    outer = fx.params(maxIter=50, relTol=1e-10, absTol=0.0)
+    inner = fx.params(maxIter=5000, relTol=1e-4, absTol=0.0)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    ir = fx.IR(ctx, A, 1.0, outer, inner)
+    x = fx.Vec(ctx, pr.A.nrow)
+    r1, in1 = ir.Solve(dev(ctx, pr.A.matvec(_sine_mode(m))), x)
+    assert r1.retcode == 0
+    x.set(0.0)
+    r2, in2 = ir.Solve(dev(ctx, pr.b), x)
+    ir.close()
+    A2 = fx.Csr.from_problem(ctx, pr.A)
+    ir2 = fx.IR(ctx, A2, 1.0, outer, inner)
+    x2 = fx.Vec(ctx, pr.A.nrow)
+    r3, in3 = ir2.Solve(dev(ctx, pr.b), x2)
+    ir2.close()
+    assert r2.retcode == r3.retcode == 0
+    assert (r2.numIter, in2) == (r3.numIter, in3) and in3 > 40, (r2.numIter, in2, r3.numIter, in3)
+    assert np.array_equal(x.numpy(), x2.numpy())
