@@ -514,37 +514,6 @@ struct BiRun {
     return dinv ? launch_iteration_t<true>(c, hout) : launch_iteration_t<false>(c, hout);
   }
 
-  bool use_graph() const { return st != nullptr; }  // the null stream cannot be captured
-
-  int build_graph() {
-    if (W.chunk[0] || !use_graph()) return 0;
-    for (int slot = 0; slot < 2; ++slot) {
-      hipGraph_t g = nullptr;
-      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int c = 0; c < kChunk && !rc; ++c)
-        rc = launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr);
-      hipError_t e = hipStreamEndCapture(st, &g);
-      if (rc) return rc;
-      FX_HIP(e);
-      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      FX_HIP(e);
-    }
-    return 0;
-  }
-
-  int launch_chunk(int slot) {
-    if (use_graph()) {
-      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
-    } else {
-      for (int c = 0; c < kChunk; ++c)
-        FX_TRY(launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr));
-    }
-    FX_HIP(hipEventRecord(W.ev[slot], st));
-    return 0;
-  }
-
   int read_state(BiState& s) {
     FX_HIP(hipStreamSynchronize(st));
     FX_HIP(hipMemcpy(&s, dstate(0), sizeof(BiState), hipMemcpyDeviceToHost));
@@ -665,7 +634,6 @@ struct BiRun {
                        (const VT*)vr(), n, W.partials);
     FX_HIP(hipGetLastError());
     FX_TRY((launch_finish<2, 0u>(st, W.partials, g, FinBiInit{dstate(0), P, W.hist})));
-    FX_TRY(build_graph());
 
     Ops o(W.ctx, A, op, S->pc, pc_dtype);
     fx_vec vb_ = nullptr, vx_ = nullptr, vr_ = nullptr, vp_ = nullptr, vpt_ = nullptr,
@@ -687,27 +655,17 @@ struct BiRun {
     for (;;) {
       ReplayPace pace;
       pace.hint = host_steps == 0 ? W.last_iters : 0;
-      SlotQueue sq;
-      for (int i = pace.initial(); i > 0; --i) {
-        FX_TRY(launch_chunk(sq.free_slot()));
-        sq.push(sq.free_slot());
-      }
-      uint64_t chunks = sq.n;
-      for (;;) {
-        const int cur = sq.front();
-        FX_HIP(hipEventSynchronize(W.ev[cur]));
-        sq.pop();
-        const BiState* hs = hslot(cur);
-        if (hs->status != ST_RUN) break;
-        FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "BiCGStab device loop did not stop");
-        const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
-        if (pace.more(hs->resAbs, target, hs->numIter, PP.minIter == 0 ? hs->denAbs : 0.0) ||
-            sq.n == 0) {
-          FX_TRY(launch_chunk(sq.free_slot()));
-          sq.push(sq.free_slot());
-          ++chunks;
-        }
-      }
+      // the iteration that ends the loop is numbered numIter - 1: a
+      // prediction of hint iterations is hint launches
+      const BiState* hs = nullptr;
+      FX_TRY(replay_loop<BiState>(
+          W, st, pace, pace.hint, max_chunks + 2,
+          [&](int c, int slot) { return launch_iteration(c, slot >= 0 ? hslot(slot) : nullptr); },
+          [&](const BiState& h) {
+            const double target = std::max(PP.relTol * h.denAbs, PP.absTol);
+            return pace.more(h.resAbs, target, h.numIter, PP.minIter == 0 ? h.denAbs : 0.0);
+          },
+          &hs, "BiCGStab"));
       FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
                "BiCGStab device loop did not stop");
       FX_TRY(read_state(s));

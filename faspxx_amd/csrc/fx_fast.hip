@@ -1024,39 +1024,6 @@ struct CgRun {
   }
   CgState* hslot(int slot) const { return (CgState*)W.hstate + slot; }
 
-  // The legacy null stream cannot be captured: replay by direct launches.
-  bool use_graph() const { return st != nullptr; }
-
-  // One graph per host slot (the last iteration copies the state there).
-  int build_graph() {
-    if (W.chunk[0] || !use_graph()) return 0;
-    for (int slot = 0; slot < 2; ++slot) {
-      hipGraph_t g = nullptr;
-      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int c = 0; c < kChunk && !rc; ++c)
-        rc = launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr);
-      hipError_t e = hipStreamEndCapture(st, &g);
-      if (rc) return rc;
-      FX_HIP(e);
-      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      FX_HIP(e);
-    }
-    return 0;
-  }
-
-  int launch_chunk(int slot) {
-    if (use_graph()) {
-      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
-    } else {
-      for (int c = 0; c < kChunk; ++c)
-        FX_TRY(launch_iteration(c, c == kChunk - 1 ? hslot(slot) : nullptr));
-    }
-    FX_HIP(hipEventRecord(W.ev[slot], st));
-    return 0;
-  }
-
   int read_state(CgState& s) {
     FX_HIP(hipStreamSynchronize(st));
     FX_HIP(hipMemcpy(&s, W.state, sizeof(CgState), hipMemcpyDeviceToHost));
@@ -1208,7 +1175,6 @@ struct CgRun {
     }
     FX_HIP(hipGetLastError());
     FX_TRY((launch_finish<2, 0u>(st, W.partials, g, FinCgInit{dstate(), P, W.hist})));
-    FX_TRY(build_graph());
 
     // Device-vector views for the host branches
     Ops o(W.ctx, A, op, S->pc, pc_dtype);
@@ -1238,27 +1204,17 @@ struct CgRun {
     for (;;) {
       ReplayPace pace;
       pace.hint = host_steps == 0 ? W.last_iters : 0;
-      SlotQueue sq;
-      for (int i = pace.initial(); i > 0; --i) {
-        FX_TRY(launch_chunk(sq.free_slot()));
-        sq.push(sq.free_slot());
-      }
-      uint64_t chunks = sq.n;
-      for (;;) {
-        const int cur = sq.front();
-        FX_HIP(hipEventSynchronize(W.ev[cur]));
-        sq.pop();
-        const CgState* hs = (const CgState*)W.hstate + cur;
-        if (hs->status != ST_RUN) break;
-        FX_CHECK(chunks <= max_chunks + 2, FX_ERROR_DEVICE, "CG device loop did not stop");
-        const double target = std::max(PP.relTol * hs->denAbs, PP.absTol);
-        if (pace.more(hs->resAbs, target, hs->numIter, PP.minIter == 0 ? hs->denAbs : 0.0) ||
-            sq.n == 0) {
-          FX_TRY(launch_chunk(sq.free_slot()));
-          sq.push(sq.free_slot());
-          ++chunks;
-        }
-      }
+      // the iteration that hands the loop's end to the host is numbered
+      // numIter - 1: a prediction of hint iterations is hint launches
+      const CgState* hs = nullptr;
+      FX_TRY(replay_loop<CgState>(
+          W, st, pace, pace.hint, max_chunks + 2,
+          [&](int c, int slot) { return launch_iteration(c, slot >= 0 ? hslot(slot) : nullptr); },
+          [&](const CgState& h) {
+            const double target = std::max(PP.relTol * h.denAbs, PP.absTol);
+            return pace.more(h.resAbs, target, h.numIter, PP.minIter == 0 ? h.denAbs : 0.0);
+          },
+          &hs, "CG"));
       FX_CHECK(++host_steps <= (uint64_t)PP.maxIter + 2, FX_ERROR_DEVICE,
                "CG device loop did not stop");
       FX_TRY(read_state(s));
@@ -1419,41 +1375,6 @@ struct IrInner {
     return 0;
   }
 
-  // The legacy null stream cannot be captured: replay by direct launches.
-  bool use_graph() const { return st != nullptr; }
-
-  // The replay graph of L iterations (even, at most kChunk) for host slot
-  // `slot`, whose last iteration copies the state there; made on first use.
-  int graph_of(int slot, int L, hipGraphExec_t* out) {
-    hipGraphExec_t& ge = L == kChunk ? W.chunk[slot] : W.tail[slot][L / 2];
-    if (!ge) {
-      hipGraph_t g = nullptr;
-      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int c = 0; c < L && !rc; ++c) rc = launch_iteration(c, c == L - 1 ? hslot(slot) : nullptr);
-      hipError_t e = hipStreamEndCapture(st, &g);
-      if (rc) return rc;
-      FX_HIP(e);
-      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      FX_HIP(e);
-    }
-    *out = ge;
-    return 0;
-  }
-
-  int launch_chunk(int slot, int L) {
-    if (use_graph()) {
-      hipGraphExec_t ge = nullptr;
-      FX_TRY(graph_of(slot, L, &ge));
-      FX_HIP(hipGraphLaunch(ge, st));
-    } else {
-      for (int c = 0; c < L; ++c) FX_TRY(launch_iteration(c, c == L - 1 ? hslot(slot) : nullptr));
-    }
-    FX_HIP(hipEventRecord(W.ev[slot], st));
-    return 0;
-  }
-
   // Replays until the device loop stops; out = the final state.  A replay
   // that finds the loop running advances it, so it stops within maxIter /
   // kChunk + 1 replays after the predicted ones.
@@ -1462,42 +1383,17 @@ struct IrInner {
   // loop is predicted to launch hint + 1; the replay predicted to end it is
   // cut to that count (rounded up to even).
   int run(uint32_t maxIter, IrState& out, uint32_t hint) {
-    const uint64_t need = hint ? (uint64_t)hint + 1 : 0;
-    // the graphs this run may replay (the full ones and the predicted last
-    // one), made before any replay is queued
-    const int tail = replay_len(need, need / kChunk * kChunk);
-    for (int slot = 0; slot < 2 && use_graph(); ++slot) {
-      hipGraphExec_t ge = nullptr;
-      FX_TRY(graph_of(slot, kChunk, &ge));
-      FX_TRY(graph_of(slot, tail, &ge));
-    }
-    const uint64_t max_chunks = (uint64_t)maxIter / kChunk + 4 + need / 2;
     ReplayPace pace;
     pace.hint = hint;
-    SlotQueue sq;
-    uint64_t queued = 0, chunks = 0;
-    auto queue = [&]() -> int {
-      const int slot = sq.free_slot(), L = replay_len(need, queued);
-      FX_TRY(launch_chunk(slot, L));
-      sq.push(slot);
-      queued += (uint64_t)L;
-      ++chunks;
-      return 0;
-    };
-    for (int i = pace.initial(); i > 0; --i) FX_TRY(queue());
-    for (;;) {
-      const int cur = sq.front();
-      FX_HIP(hipEventSynchronize(W.ev[cur]));
-      sq.pop();
-      const IrState* hs = hslot(cur);
-      if (hs->status != ST_RUN) {
-        out = *hs;
-        return 0;
-      }
-      FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "IR inner loop did not stop");
-      if (pace.more(hs->gamma, hs->thr, hs->numIter, hs->gamma0) || sq.n == 0)  // squared
-        FX_TRY(queue());
-    }
+    const uint64_t need = hint ? (uint64_t)hint + 1 : 0;
+    const IrState* hs = nullptr;
+    FX_TRY(replay_loop<IrState>(
+        W, st, pace, need, (uint64_t)maxIter / kChunk + 4 + need / 2,
+        [&](int c, int slot) { return launch_iteration(c, slot >= 0 ? hslot(slot) : nullptr); },
+        [&](const IrState& h) { return pace.more(h.gamma, h.thr, h.numIter, h.gamma0); },  // squared
+        &hs, "IR inner"));
+    out = *hs;
+    return 0;
   }
 };
 

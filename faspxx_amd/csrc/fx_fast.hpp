@@ -181,6 +181,98 @@ inline FastWs::~FastWs() {
 }
 
 // ---------------------------------------------------------------------------
+// Replay loops
+// ---------------------------------------------------------------------------
+// A fused solver's device loop is replayed from graphs of kChunk iterations
+// (or fewer, see replay_len), one graph per host slot: iteration c is
+// enqueued by iter(c, slot), where slot >= 0 for the replay's last iteration
+// (it copies the state to that host slot) and -1 otherwise.
+
+// The replay graph of L iterations for host slot `slot`, made on first use
+// and kept in W.
+template <class Iter>
+int replay_graph(FastWs& W, hipStream_t st, int slot, int L, Iter& iter, hipGraphExec_t* out) {
+  hipGraphExec_t& ge = L == kChunk ? W.chunk[slot] : W.tail[slot][L / 2];
+  if (!ge) {
+    hipGraph_t g = nullptr;
+    FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int c = 0; c < L && !rc; ++c) rc = iter(c, c == L - 1 ? slot : -1);
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    FX_HIP(e);
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FX_HIP(e);
+  }
+  *out = ge;
+  return 0;
+}
+
+// One replay of L iterations on host slot `slot`, then the slot's event.
+// The legacy null stream cannot be captured: there the iterations are
+// launched directly.
+template <class Iter>
+int replay_launch(FastWs& W, hipStream_t st, int slot, int L, Iter& iter) {
+  if (st) {
+    hipGraphExec_t ge = nullptr;
+    FX_TRY(replay_graph(W, st, slot, L, iter, &ge));
+    FX_HIP(hipGraphLaunch(ge, st));
+  } else {
+    for (int c = 0; c < L; ++c) FX_TRY(iter(c, c == L - 1 ? slot : -1));
+  }
+  FX_HIP(hipEventRecord(W.ev[slot], st));
+  return 0;
+}
+
+// Replays the device loop until the state a replay leaves in its host slot
+// is no longer ST_RUN; *out is then that host copy.  pace.initial() replays
+// go out at once; after each finished replay another goes out when more(s)
+// asks for it (s: the state it left) or none is in flight.  need: the
+// iterations the loop is predicted to launch before it stops (0: no
+// prediction); the replay predicted to end the loop is cut to that count.
+// At most max_chunks replays may find the loop running.
+template <class State, class Iter, class More>
+int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need,
+                uint64_t max_chunks, Iter&& iter, More&& more, const State** out,
+                const char* what) {
+  if (st) {  // the graphs this loop may replay, made before any replay is queued
+    const int tail = replay_len(need, need / kChunk * kChunk);
+    for (int slot = 0; slot < 2; ++slot) {
+      hipGraphExec_t ge = nullptr;
+      FX_TRY(replay_graph(W, st, slot, kChunk, iter, &ge));
+      FX_TRY(replay_graph(W, st, slot, tail, iter, &ge));
+    }
+  }
+  SlotQueue sq;
+  uint64_t queued = 0, chunks = 0;
+  auto queue = [&]() -> int {
+    const int slot = sq.free_slot(), L = replay_len(need, queued);
+    FX_TRY(replay_launch(W, st, slot, L, iter));
+    sq.push(slot);
+    queued += (uint64_t)L;
+    ++chunks;
+    return 0;
+  };
+  for (int i = pace.initial(); i > 0; --i) FX_TRY(queue());
+  for (;;) {
+    const int cur = sq.front();
+    FX_HIP(hipEventSynchronize(W.ev[cur]));
+    sq.pop();
+    const State* hs = (const State*)W.hstate + cur;
+    if (hs->status != ST_RUN) {
+      *out = hs;
+      return 0;
+    }
+    FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "%s device loop did not stop", what);
+    if (more(*hs) || sq.n == 0) FX_TRY(queue());
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The operator of a fused solve
 // ---------------------------------------------------------------------------
 // A matrix: its SpMV with the solver's epilogue in the same kernel.  A
