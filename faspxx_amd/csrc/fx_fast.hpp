@@ -12,7 +12,7 @@ namespace fx {
 // Iterations per graph replay (even: the state's parity buffers restart at
 // 0 with every replay).
 #ifndef FX_CHUNK
-#define FX_CHUNK 16
+#define FX_CHUNK 32
 #endif
 constexpr int kChunk = FX_CHUNK;
 static_assert(kChunk % 2 == 0, "replays hold an even number of iterations");
@@ -72,8 +72,9 @@ void gmres_ws_free(void* p);
 
 // Replay loops keep two replays in flight, so that the GPU never waits for
 // the host between them; but the one behind the replay that ends the loop
-// runs only skipped kernels (16 iterations' worth: about 56 us for the
-// FP64 CG).  After each finished replay this predicts, from how fast a
+// runs only skipped kernels (a replay's worth, about 3.5 us per iteration
+// for the FP64 CG on the 128^3 stencil).  After each finished replay this
+// predicts, from how fast a
 // measure that the loop drives below a target (a residual norm) fell over
 // the last replay, whether the replay still in flight ends the loop, in
 // which case no other is queued yet.
