@@ -567,7 +567,7 @@ __global__ void k_jac_flags(int* flags, const double* xinf) {
   flags[2] = 0;
 }
 
-constexpr int kJacChunk = 6;  // sweeps per replay: a multiple of 3 buffers and 2 states
+constexpr int kJacPeriod = 6;  // the sweeps' buffer rotation: 3 x buffers and 2 states
 
 // Final residual norms (src/Iter.cxx:102-109): ||b - A x||_2 and _inf.
 struct EpiResNorms {
@@ -1245,7 +1245,7 @@ struct CgRun {
 // ---------------------------------------------------------------------------
 // Jacobi solver
 // ---------------------------------------------------------------------------
-int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_result* res) {
+int jacobi_run(fx_solver S, FastWs& W, const double* b_user, double* x_user, fx_result* res) {
   const fx_params& PP = S->prm;
   const CgPrm P = make_prm(PP);
   hipStream_t st = W.ctx->stream;
@@ -1257,6 +1257,10 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   const unsigned G = spmv_grid_of<double>(A, W.ctx->sms);
   double* part[2] = {W.partials, W.partials + 2 * (size_t)G};
   FX_HIP(hipMemcpyAsync(xbuf[0], x_user, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  // the replay graphs read b from the workspace: a later solve may pass
+  // another vector
+  const double* b = (const double*)W.p;
+  FX_HIP(hipMemcpyAsync(W.p, b_user, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   JacLagState s0{};
   s0.resAbs = 1.0;
   s0.resRel = 1.0;
@@ -1267,65 +1271,43 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b, double* x_user, fx_resul
   hipLaunchKernelGGL(k_jac_flags, dim3(1), dim3(1), 0, st, flags, (const double*)W.dscal);
   FX_HIP(hipGetLastError());
   const double* dinv = S->pc->dinv;
-  auto sweeps = [&](int slot) -> int {
-    for (int c = 0; c < kJacChunk; ++c) {
-      const EpiJacobiLag e{xbuf[(c + 1) % 3], xbuf[c % 3], b, dinv, flags + c % 3,
-                           flags + (c + 1) % 3, flags + (c + 2) % 3, ds + c % 2, ds + (c + 1) % 2,
-                           part[(c + 1) % 2], G, P, (uint32_t)((c + 2) % 3), (uint32_t)(c % 3),
-                           W.hist, 0};
-      FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
-          A, st, XLoad<double>{xbuf[c % 3]}, BLoad<double>{b}, e, part[c % 2], nullptr,
-          nullptr, false)));
+  // sweep c: its buffers rotate with c % 3 (x) and c % 2 (state), so every
+  // replay holds a multiple of 6 sweeps and starts on x buffer 0 and state
+  // buffer 0; the last sweep of a replay is followed by the copy of the
+  // state to the replay's host slot
+  auto sweep = [&](int c, int slot) -> int {
+    const EpiJacobiLag e{xbuf[(c + 1) % 3], xbuf[c % 3], b, dinv, flags + c % 3,
+                         flags + (c + 1) % 3, flags + (c + 2) % 3, ds + c % 2, ds + (c + 1) % 2,
+                         part[(c + 1) % 2], G, P, (uint32_t)((c + 2) % 3), (uint32_t)(c % 3),
+                         W.hist, 0};
+    FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
+        A, st, XLoad<double>{xbuf[c % 3]}, BLoad<double>{b}, e, part[c % 2], nullptr, nullptr,
+        false)));
+    if (slot >= 0) {
+      hipLaunchKernelGGL(k_jac_report, dim3(1), dim3(64), 0, st, (const JacLagState*)ds,
+                         (JacLagState*)W.hstate + slot);
+      FX_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_jac_report, dim3(1), dim3(64), 0, st, (const JacLagState*)ds,
-                       (JacLagState*)W.hstate + slot);
-    FX_HIP(hipGetLastError());
     return 0;
   };
-  const bool graph = st != nullptr;  // the legacy null stream cannot be captured
-  if (graph && !W.chunk[0]) {
-    for (int slot = 0; slot < 2; ++slot) {
-      hipGraph_t g = nullptr;
-      FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      const int rc = sweeps(slot);
-      hipError_t e = hipStreamEndCapture(st, &g);
-      if (rc) return rc;
-      FX_HIP(e);
-      e = hipGraphInstantiate(&W.chunk[slot], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      FX_HIP(e);
-    }
-  }
-  auto replay = [&](int slot) -> int {
-    if (graph) {
-      FX_HIP(hipGraphLaunch(W.chunk[slot], st));
-    } else {
-      FX_TRY(sweeps(slot));
-    }
-    FX_HIP(hipEventRecord(W.ev[slot], st));
-    return 0;
-  };
-  // kJacChunk is a multiple of 6: every replay starts on x buffer 0 and
-  // state buffer 0, and ends with the state in buffer 0, which its last
-  // kernel copies to the replay's host slot
+  // Sweep m decides sweep m - 1, so the loop launches maxIter + 1 sweeps at
+  // most, and a run that met the tolerance after numIter sweeps launched
+  // numIter + 2: the last run's count predicts this one.
+  const uint64_t max_launch = (uint64_t)PP.maxIter + 1;
+  const uint64_t need =
+      W.last_iters ? std::min<uint64_t>((uint64_t)W.last_iters + 2, max_launch) : max_launch;
+  ReplayPace pace;  // two replays in flight while the loop may run on
+  const JacLagState* hs = nullptr;
+  FX_TRY(replay_loop<JacLagState>(
+      W, st, pace, need, max_launch / kJacPeriod + 4, sweep,
+      [](const JacLagState&) { return true; }, &hs, "Jacobi", kJacPeriod, max_launch));
   JacLagState s{};
-  int cur = 0;
-  for (int i = 0; i < 2; ++i) FX_TRY(replay(i));
-  const uint64_t max_chunks = (uint64_t)PP.maxIter / kJacChunk + 4;
-  uint64_t chunks = 2;
-  for (;;) {
-    FX_HIP(hipEventSynchronize(W.ev[cur]));
-    if (((const JacLagState*)W.hstate + cur)->status != ST_RUN) break;
-    FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "Jacobi device loop did not stop");
-    ++chunks;
-    FX_TRY(replay(cur));
-    cur ^= 1;
-  }
   FX_HIP(hipStreamSynchronize(st));
   FX_HIP(hipMemcpy(&s, ds, sizeof(JacLagState), hipMemcpyDeviceToHost));
   FX_CHECK(s.answer < 3, FX_ERROR_DEVICE, "Jacobi answer buffer %u", s.answer);
   double* ans = xbuf[s.answer];
   res->numIter = s.numIter;
+  W.last_iters = s.numIter;
   if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
     FX_TRY(norminf_to(st, ans, n, W.dscal, W.partials));
     FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(

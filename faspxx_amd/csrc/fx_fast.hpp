@@ -119,13 +119,16 @@ struct SlotQueue {
   }
 };
 
-// Iterations of the next replay: kChunk, or for the replay that is predicted
-// to end the loop (need: the iterations the loop is predicted to launch, 0:
-// no prediction; queued: those queued so far) the even count that covers the
-// rest.  Even, so that every later replay starts on the same state parity.
-inline int replay_len(uint64_t need, uint64_t queued) {
-  if (need == 0 || queued >= need || need - queued >= (uint64_t)kChunk) return kChunk;
-  return (int)((need - queued + 1) & ~(uint64_t)1);
+// Iterations of the next replay: the most that fit kChunk, or for the
+// replay that is predicted to end the loop (need: the iterations the loop
+// is predicted to launch, 0: no prediction; queued: those queued so far) the
+// count that covers the rest.  Both are multiples of `unit`, the period of
+// the loop's buffer rotation (2 for the state's parity buffers), so that
+// every replay starts on the same buffers.
+inline int replay_len(uint64_t need, uint64_t queued, int unit = 2) {
+  const int full = kChunk / unit * unit;
+  if (need == 0 || queued >= need || need - queued >= (uint64_t)full) return full;
+  return (int)((need - queued + unit - 1) / unit * unit);
 }
 
 // ---------------------------------------------------------------------------
@@ -235,30 +238,34 @@ int replay_launch(FastWs& W, hipStream_t st, int slot, int L, Iter& iter) {
 // asks for it (s: the state it left) or none is in flight.  need: the
 // iterations the loop is predicted to launch before it stops (0: no
 // prediction); the replay predicted to end the loop is cut to that count.
-// At most max_chunks replays may find the loop running.
+// unit: see replay_len.  max_launch: iterations after which the loop has
+// certainly stopped; no replay is queued beyond them while one is in
+// flight.  At most max_chunks replays may find the loop running.
 template <class State, class Iter, class More>
 int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need,
                 uint64_t max_chunks, Iter&& iter, More&& more, const State** out,
-                const char* what) {
+                const char* what, int unit = 2, uint64_t max_launch = UINT64_MAX) {
+  const int full = replay_len(0, 0, unit);
   if (st) {  // the graphs this loop may replay, made before any replay is queued
-    const int tail = replay_len(need, need / kChunk * kChunk);
+    const int tail = replay_len(need, need / full * full, unit);
     for (int slot = 0; slot < 2; ++slot) {
       hipGraphExec_t ge = nullptr;
-      FX_TRY(replay_graph(W, st, slot, kChunk, iter, &ge));
+      FX_TRY(replay_graph(W, st, slot, full, iter, &ge));
       FX_TRY(replay_graph(W, st, slot, tail, iter, &ge));
     }
   }
   SlotQueue sq;
   uint64_t queued = 0, chunks = 0;
   auto queue = [&]() -> int {
-    const int slot = sq.free_slot(), L = replay_len(need, queued);
+    const int slot = sq.free_slot(), L = replay_len(need, queued, unit);
     FX_TRY(replay_launch(W, st, slot, L, iter));
     sq.push(slot);
     queued += (uint64_t)L;
     ++chunks;
     return 0;
   };
-  for (int i = pace.initial(); i > 0; --i) FX_TRY(queue());
+  FX_TRY(queue());
+  if (pace.initial() > 1 && queued < max_launch) FX_TRY(queue());
   for (;;) {
     const int cur = sq.front();
     FX_HIP(hipEventSynchronize(W.ev[cur]));
@@ -269,7 +276,7 @@ int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need
       return 0;
     }
     FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "%s device loop did not stop", what);
-    if (more(*hs) || sq.n == 0) FX_TRY(queue());
+    if ((more(*hs) && queued < max_launch) || sq.n == 0) FX_TRY(queue());
   }
 }
 

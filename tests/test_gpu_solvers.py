@@ -155,6 +155,53 @@ def test_jacobi_sweeps_bitexact(ctx, oracle, problems, force_ops):
     assert rel_err(hist, o["hist"][:50]) < 1e-13
 
 
+@pytest.mark.parametrize("maxIter", [1, 4, 29, 30, 31, 59, 60, 61])
+def test_jacobi_replay_lengths(ctx, oracle, problems, maxIter):
+    """Sweep counts around the replay lengths (sweeps are replayed in
+    multiples of 6, up to 30 at a time; a run of maxIter sweeps launches
+    maxIter + 1 and its last replay is cut to cover just those): bit-exact
+    against the oracle, and a second solve on the same solver (which
+    predicts its sweeps from the first) gives the same bits."""
+    pr = problems(2, 32)
+    M = Matrix.of(pr.A)
+    prm = dict(maxIter=maxIter, relTol=1e-30, absTol=0.0)
+    o = oracle.solve(JACOBI, M, pr.b, np.zeros(M.nrow), weight=2 / 3, hist_cap=80, **prm)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    s = fx.Solver(ctx, fx.JACOBI)
+    s.Setup(A, fx.Precond(ctx, "jacobi", A, 2 / 3, 1), fx.params(**prm))
+    for k in range(2):
+        x = dev(ctx, np.zeros(M.nrow))
+        res = s.Solve(dev(ctx, pr.b), x)
+        assert s.last_path == "fused"
+        assert res.numIter == o["numIter"] == maxIter, (k, res.numIter, o["numIter"])
+        assert np.array_equal(x.numpy(), o["x"])
+
+
+@pytest.mark.parametrize("relTol", [1e-3, 1e-4])
+def test_jacobi_tolerance_repeat(ctx, oracle, problems, relTol):
+    """Weighted Jacobi stopped by the tolerance (36 and 128 sweeps at 32^3):
+    the count within 1 of the oracle's and x bit-exact when the counts
+    agree; a second solve on the same solver, whose last replay is cut to
+    the first solve's count, gives the same count and bits."""
+    pr = problems(2, 32)
+    M = Matrix.of(pr.A)
+    prm = dict(maxIter=2000, relTol=relTol, absTol=0.0)
+    o = oracle.solve(JACOBI, M, pr.b, np.zeros(M.nrow), weight=2 / 3, hist_cap=2100, **prm)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    s = fx.Solver(ctx, fx.JACOBI)
+    s.Setup(A, fx.Precond(ctx, "jacobi", A, 2 / 3, 1), fx.params(**prm))
+    out = []
+    for _ in range(2):
+        x = dev(ctx, np.zeros(M.nrow))
+        res = s.Solve(dev(ctx, pr.b), x)
+        assert s.last_path == "fused" and res.retcode == 0
+        out.append((res.numIter, x.numpy()))
+    assert abs(out[0][0] - o["numIter"]) <= 1, (out[0][0], o["numIter"])
+    if out[0][0] == o["numIter"]:
+        assert np.array_equal(out[0][1], o["x"])
+    assert out[1][0] == out[0][0] and np.array_equal(out[1][1], out[0][1])
+
+
 def test_jacobi_converges_unit(ctx):
     """UnitTestsJacobi 3x3 (utest/src/UnitTestsJacobi.cxx:85-157)."""
     class PR:
