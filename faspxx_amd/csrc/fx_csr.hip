@@ -903,6 +903,9 @@ int fx_csr_set_values_dev(fx_csr A, const double* dev_values) {
   if (dev_values != A->va)
     FX_HIP(hipMemcpyAsync(A->va, dev_values, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice,
                           A->ctx->stream));
+  // a structure-only matrix that gains values is another operator to a
+  // solver set up on it (its replay graphs read no values): a new id
+  if (!A->has_values) A->gen = next_handle_gen();
   A->has_values = true;
   if (A->sell) FX_TRY(sell_fill64(A, A->va, A->sval));
   if (A->sorted) FX_TRY(sorted_fill(A));
@@ -1082,9 +1085,8 @@ int fx_csr_transpose_update(fx_ctx ctx, fx_csr A, fx_csr At) {
   hipLaunchKernelGGL(k_gather_values, dim3(stream_blocks(A->nnz)), dim3(kThreads), 0, ctx->stream,
                      At->src, A->va, At->va, A->nnz);
   FX_HIP(hipGetLastError());
-  At->has_values = true;
-  if (At->va32) return fx_csr_make_f32(At);
-  return 0;
+  // the copies of the values (sliced, sorted, f32) follow At->va
+  return fx_csr_set_values_dev(At, At->va);
 }
 
 // g_k = -lam[row(k)] * x[col(k)] for every stored entry k: the gradient of a

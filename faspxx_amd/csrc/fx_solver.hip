@@ -888,6 +888,13 @@ int fx_solver_solve(fx_solver s, fx_vec b, fx_vec x, fx_result* res) {
            x->n, n);
   FX_CHECK(b->dtype == FX_F64 && x->dtype == FX_F64, FX_ERROR_INPUT_PAR,
            "solvers take f64 vectors (use fx_solve_ir for f32 inner solves)");
+  // a bound matrix that has become another operator since set-up (a
+  // structure-only one that gained values) gets new replay graphs
+  if (s->A && (s->A->gen != s->A_gen || (s->pc && s->pc->gen != s->pc_gen))) {
+    s->release_fast();
+    s->A_gen = s->A->gen;
+    if (s->pc) s->pc_gen = s->pc->gen;
+  }
   res->retcode = 0;
   res->numIter = 0;
   res->norm2 = 1e60;  // SOL() initialises norm2 / normInf to LARGE_DBL

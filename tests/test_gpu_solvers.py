@@ -643,3 +643,29 @@ def test_ir_solve_after_short_solve(ctx, problems):
     assert r2.retcode == r3.retcode == 0
     assert (r2.numIter, in2) == (r3.numIter, in3) and in3 > 40, (r2.numIter, in2, r3.numIter, in3)
     assert np.array_equal(x.numpy(), x2.numpy())
+
+
+@pytest.mark.parametrize("kind", [fx.CG])
+def test_solve_after_values_set(ctx, kind):
+    """A solver set up on a structure-only matrix (every stored value 1,
+    here the identity's pattern) and solved once, then given values: its
+    next solve uses them, since the matrix is then another operator to it
+    (its replay graphs were made for a matrix without values)."""
+    n = 20000
+    idx = np.arange(n, dtype=np.uint32)
+    A = fx.Csr(ctx, n, n, np.arange(n + 1, dtype=np.uint32), idx)
+    P = fx.Precond(ctx, "identity", A, 1.0, 1)
+    s = fx.Solver(ctx, kind)
+    s.Setup(A, P, fx.params(maxIter=100, relTol=1e-12, absTol=0.0))
+    b = np.random.default_rng(5).uniform(1.0, 2.0, n)
+    x = dev(ctx, np.zeros(n))
+    r = s.Solve(dev(ctx, b), x)
+    assert s.last_path == "fused" and r.retcode == 0
+    assert rel_err(x.numpy(), b) < 1e-12
+    d = 2.0 + (np.arange(n) % 3)
+    vals = dev(ctx, d)
+    A.set_values_dev(vals.data_ptr)
+    x = dev(ctx, np.zeros(n))
+    r = s.Solve(dev(ctx, b), x)
+    assert s.last_path == "fused" and r.retcode == 0
+    assert rel_err(x.numpy(), b / d) < 1e-12
