@@ -337,10 +337,11 @@ int fx_op_create(fx_ctx ctx, uint32_t nrow, uint32_t mcol, fx_op_apply_fn apply,
  * and enqueues the same work on every call, e.g. a device stencil kernel
  * for the MatFree::Apply of example/TestMatFree.cxx:86-176.  The Krylov
  * solvers then run their fused form on it (the identity or a one-sweep
- * Jacobi preconditioner): apply is called while the replay graph of the
- * first Solve after fx_solver_setup_op / fx_solver_update_op is captured,
- * and its device work is replayed from then on.  No counterpart in the
- * reference, whose LOP::Apply is a host call. */
+ * Jacobi preconditioner): apply is called while replay graphs are captured
+ * (on the first Solve after fx_solver_setup_op / fx_solver_update_op, and
+ * on a later Solve for each shorter last replay it first needs), and its
+ * device work is replayed from then on.  No counterpart in the reference,
+ * whose LOP::Apply is a host call. */
 int fx_op_set_device_only(fx_op op, int device_only);
 int fx_op_destroy(fx_op op);
 int fx_solver_setup_op(fx_solver s, fx_op A, fx_pc pc, const fx_params* prm);
@@ -388,9 +389,10 @@ int fx_solve_ir(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_pa
                 uint32_t* inner_iters);
 /* The same as a reusable object: create allocates the workspace (the f32
  * values of S A S and S, or an f32 value copy and the Jacobi diagonal;
- * inner-solver buffers); solve allocates nothing except on its first call,
- * which also builds the replay graphs.  Values of A changed after creation
- * are not seen. */
+ * inner-solver buffers); solve allocates nothing except the replay graphs
+ * it makes on first use: the full replays on the first call, and a shorter
+ * last replay per predicted length on later calls.  Values of A changed
+ * after creation are not seen by the inner solves. */
 int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_params* outer,
                  const fx_params* inner, fx_ir* out);
 int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner_iters);
