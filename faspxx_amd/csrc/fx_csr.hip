@@ -72,8 +72,8 @@ __global__ void k_get_diag(const double* __restrict__ va, const uint32_t* __rest
 
 // cd[k] = ci[k] - r0 for the entries of row block b (one workgroup per
 // block); *bad is set when a delta does not fit in 16 bits.
-__global__ void k_col_delta(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
-                            const uint32_t* __restrict__ blk, uint32_t nblk,
+__global__ void k_col_delta(const uint32_t* __restrict__ ci, const uint32_t* __restrict__ blk,
+                            uint32_t nblk,
                             int16_t* __restrict__ cd, uint32_t* bad) {
   const uint32_t b = blockIdx.x;
   if (b >= nblk) return;
@@ -622,7 +622,7 @@ int csr_set_blocks(fx_csr A, const uint32_t* host_rp, const uint32_t* host_ci) {
     uint32_t hbad = 0;
     hipError_t err = hipMemsetAsync(bad, 0, sizeof(uint32_t), st);
     if (err == hipSuccess) {
-      hipLaunchKernelGGL(k_col_delta, dim3(A->nblk), dim3(kThreads), 0, st, A->rp, A->ci, A->blk,
+      hipLaunchKernelGGL(k_col_delta, dim3(A->nblk), dim3(kThreads), 0, st, A->ci, A->blk,
                          A->nblk, A->cd, bad);
       err = hipGetLastError();
     }

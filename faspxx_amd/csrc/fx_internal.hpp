@@ -433,7 +433,7 @@ __device__ __forceinline__ void fin_load(const double* __restrict__ partials, un
     double a = ident;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const double x = (tid + j * NT < G) ? v[k][j] : ident;
+      const double x = ((unsigned)(tid + j * NT) < G) ? v[k][j] : ident;
       a = mx ? fmax(a, x) : a + x;
     }
     acc[k] = a;
