@@ -653,7 +653,17 @@ struct IrState {
   uint32_t numIter, maxIter;
   int status, reason;
 };
-enum { IR_CONVERGED = 0, IR_MAXIT = 1, IR_BREAKDOWN = 2 };
+// IR_OUTER: the outer loop met its tolerance at the residual before this
+// inner solve, which therefore does not run
+enum { IR_CONVERGED = 0, IR_MAXIT = 1, IR_BREAKDOWN = 2, IR_OUTER = 3 };
+
+// The outer loop's record after each residual, in pinned host memory:
+// ||b - A x||, the reference value of the relative test, and whether the
+// loop stops there.
+struct IrOuterRec {
+  double resAbs, denAbs;
+  int done, pad;
+};
 
 // s_i = 1 / sqrt(a_ii); *bad = 1 where row i has no diagonal entry or
 // a_ii is not positive and finite.
@@ -697,22 +707,38 @@ struct EpiResidScaled {
   __device__ void fin(double (&)[2]) const {}
 };
 
-// ||t|| for the host, and the inner solve's starting state.
+// The outer loop's record (||t||, its test against the outer tolerance; the
+// first residual of a solve sets the reference value), and the inner
+// solve's starting state: running, or stopped (IR_OUTER) when the outer
+// loop is done, so that an inner solve queued before the host has looked at
+// the record does not run.
 struct FinIrInit {
-  double* out;
+  IrOuterRec* rec;  // pinned
+  double* den;      // the outer loop's reference value, on the device
   IrState* st;
-  double relTol, absTol;
+  double relTol, absTol;    // inner
+  double orelTol, oabsTol;  // outer
   uint32_t maxIter;
+  int first;
   __device__ bool skip() const { return false; }
   __device__ void fin(double (&s)[2]) const {
     if (threadIdx.x != 0) return;
-    out[0] = sqrt(s[0]);
+    const double resAbs = sqrt(s[0]);
+    if (first) *den = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+    const double denAbs = *den;
+    const bool done = resAbs / denAbs < orelTol || resAbs < oabsTol;
     IrState S{};
     S.gamma0 = s[1];
     S.thr = fmax(relTol * relTol * s[1], absTol * absTol);
     S.maxIter = maxIter;
-    S.status = ST_RUN;
+    S.status = done ? ST_DONE : ST_RUN;
+    S.reason = done ? IR_OUTER : IR_CONVERGED;
     *st = S;
+    IrOuterRec R{};
+    R.resAbs = resAbs;
+    R.denAbs = denAbs;
+    R.done = done ? 1 : 0;
+    *rec = R;
   }
 };
 
@@ -1478,6 +1504,8 @@ struct fx_ir_s {
   float* vs32 = nullptr;
   float* vs32o = nullptr;  // vs32 in a sorted matrix's row order
   std::vector<uint32_t> inner_counts;  // each outer step's inner iterations, last solve
+  uint32_t outer_last = 0;             // outer steps of the last solve (0: none)
+  IrOuterRec* hrec = nullptr;          // pinned
   int* bad = nullptr;
   uint32_t* blk512 = nullptr;  // row-pattern matrices: As's own 512-row blocks
   uint16_t* rpb512 = nullptr;
@@ -1490,6 +1518,7 @@ struct fx_ir_s {
     (void)hipFree(r32);
     (void)hipFree(d32);
     (void)hipFree(dsc);
+    if (hrec) (void)hipHostFree(hrec);
     (void)hipFree(part);
     (void)hipFree(sc);
     (void)hipFree(vs32);
@@ -1602,6 +1631,7 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
                          kMaxRedBlocks);
   if (hipMalloc(&ir->r32, sizeof(float) * nb) || hipMalloc(&ir->d32, sizeof(float) * nb) ||
       hipMalloc(&ir->dsc, sizeof(double) * 4) ||
+      hipHostMalloc(&ir->hrec, sizeof(IrOuterRec), hipHostMallocDefault) ||
       hipMalloc(&ir->part, sizeof(double) * np)) {
     set_error("IR workspace allocation failed");
     return fail(FX_ERROR_ALLOC_MEM);
@@ -1643,27 +1673,48 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
             EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc}, ir->part, nullptr,
             nullptr, false);
         if (!rc)
-          rc = launch_finish<2, 0u>(st, ir->part, spmv_grid_of<double>(A, ctx->sms),
-                                    FinIrInit{ir->dsc, (IrState*)ir->W->state, ir->inner.relTol,
-                                              ir->inner.absTol, ir->inner.maxIter});
+          rc = launch_finish<2, 0u>(
+              st, ir->part, spmv_grid_of<double>(A, ctx->sms),
+              FinIrInit{ir->hrec, ir->dsc + 1, (IrState*)ir->W->state, ir->inner.relTol,
+                        ir->inner.absTol, outer.relTol, outer.absTol, ir->inner.maxIter,
+                        outer_it == 0 ? 1 : 0});
+        if (rc) break;
+        // When the last solve went on past this residual, the outer loop is
+        // predicted to go on: the inner solve is queued at once, without
+        // waiting for the residual, and FinIrInit stops it on the device if
+        // the outer loop is in fact done.
+        const bool ahead = outer_it < ir->outer_last && outer_it < outer.maxIter;
+        if (!ahead) {
+          if (hipStreamSynchronize(st)) {
+            rc = FX_ERROR_DEVICE;
+            break;
+          }
+          resAbs = ir->hrec->resAbs;
+          denAbs = ir->hrec->denAbs;
+          if (ir->hrec->done) break;
+          if (outer_it >= outer.maxIter) {
+            err = FX_ERROR_SOLVER_MAXIT;
+            break;
+          }
+        }
       } else {
         rc = launch_spmv<double, double, ROW_AX, 1, 0u>(
             A, st, XLoad<double>{(const double*)x->d}, BNone{},
             EpiResidF32{ir->r32, (const double*)b->d, ir->dsc}, ir->part);
-      }
-      if (rc) break;
-      double h = 0;
-      if (hipMemcpyAsync(&h, ir->dsc, sizeof(double), hipMemcpyDeviceToHost, st) ||
-          hipStreamSynchronize(st)) {
-        rc = FX_ERROR_DEVICE;
-        break;
-      }
-      resAbs = h;
-      if (outer_it == 0) denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
-      if (resAbs / denAbs < outer.relTol || resAbs < outer.absTol) break;
-      if (outer_it >= outer.maxIter) {
-        err = FX_ERROR_SOLVER_MAXIT;
-        break;
+        if (rc) break;
+        double h = 0;
+        if (hipMemcpyAsync(&h, ir->dsc, sizeof(double), hipMemcpyDeviceToHost, st) ||
+            hipStreamSynchronize(st)) {
+          rc = FX_ERROR_DEVICE;
+          break;
+        }
+        resAbs = h;
+        if (outer_it == 0) denAbs = (kCloseZero > resAbs) ? kCloseZero : resAbs;
+        if (resAbs / denAbs < outer.relTol || resAbs < outer.absTol) break;
+        if (outer_it >= outer.maxIter) {
+          err = FX_ERROR_SOLVER_MAXIT;
+          break;
+        }
       }
       ++outer_it;
       if (ir->scaled) {
@@ -1675,6 +1726,10 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         const uint32_t hint = step < ir->inner_counts.size() ? ir->inner_counts[step] : 0u;
         rc = in.run(ir->inner.maxIter, fs, hint);
         if (rc) break;
+        if (fs.reason == IR_OUTER) {  // queued ahead, but the outer loop was done
+          --outer_it;
+          break;
+        }
         if (ir->inner_counts.size() <= step) ir->inner_counts.resize(step + 1, 0u);
         ir->inner_counts[step] = fs.reason == IR_CONVERGED ? fs.numIter : 0;
         inner_total += fs.numIter;
@@ -1724,6 +1779,11 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
   }
   (void)hipEventRecord(e1, st);
   (void)hipEventSynchronize(e1);
+  if (ir->scaled) {
+    // the last residual's record (the host did not wait for every one)
+    if (ir->hrec) resAbs = ir->hrec->resAbs;
+    ir->outer_last = (rc || err) ? 0u : outer_it;
+  }
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);

@@ -669,3 +669,31 @@ def test_solve_after_values_set(ctx, kind):
     r = s.Solve(dev(ctx, b), x)
     assert s.last_path == "fused" and r.retcode == 0
     assert rel_err(x.numpy(), b / d) < 1e-12
+
+
+def test_ir_outer_steps_queued_ahead(ctx, problems):
+    """The refinement queues an inner solve before the host has looked at
+    the residual that precedes it when the last solve went on past that
+    residual; the residual's finisher stops it on the device when the outer
+    loop is in fact done.  A zero right-hand side (done at once, with x left
+    at 0) after a solve of several outer steps takes that path; the solves
+    of the real right-hand side before and after it, the last one queued
+    ahead again, agree bit for bit."""
+    pr = problems(2, 32)
+    n = pr.A.nrow
+    outer = fx.params(maxIter=50, relTol=1e-10, absTol=0.0)
+    inner = fx.params(maxIter=5000, relTol=1e-4, absTol=0.0)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    ir = fx.IR(ctx, A, 1.0, outer, inner)
+    runs = []
+    for rhs in (pr.b, np.zeros(n), pr.b, pr.b):
+        x = fx.Vec(ctx, n)
+        res, inner_its = ir.Solve(dev(ctx, rhs), x)
+        assert res.retcode == 0
+        runs.append((res.numIter, inner_its, x.numpy()))
+    ir.close()
+    assert runs[0][0] >= 2
+    assert runs[1][0] == 0 and runs[1][1] == 0 and not np.any(runs[1][2])
+    for k in (2, 3):
+        assert runs[k][:2] == runs[0][:2], (runs[k][:2], runs[0][:2])
+        assert np.array_equal(runs[k][2], runs[0][2])
