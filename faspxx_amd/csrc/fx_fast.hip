@@ -657,6 +657,12 @@ struct IrState {
 // inner solve, which therefore does not run
 enum { IR_CONVERGED = 0, IR_MAXIT = 1, IR_BREAKDOWN = 2, IR_OUTER = 3 };
 
+// True while the inner solve whose state is cs has not run to its end: it
+// is running, or it was stopped before it began (IR_OUTER).
+__device__ __forceinline__ bool ir_inner_open(const IrState* cs) {
+  return cs->status == ST_RUN || cs->reason == IR_OUTER;
+}
+
 // The outer loop's record after each residual, in pinned host memory:
 // ||b - A x||, the reference value of the relative test, and whether the
 // loop stops there.
@@ -692,11 +698,14 @@ __global__ void k_ir_scale_vals(const uint32_t* __restrict__ rp, const uint32_t*
 
 // Outer residual for the scaled inner solve: t = b - A x in f64 (Apply +
 // XPAY order), rhat = (float)(s_i t_i); partials of (t, t) and (rhat, rhat).
+// cs: when set, the step runs only if that inner solve has run and stopped
+// (it was queued before the host knew).
 struct EpiResidScaled {
   float* __restrict__ r32;
   const double* __restrict__ b;
   const double* __restrict__ sc;
-  __device__ bool skip() const { return false; }
+  const IrState* cs;
+  __device__ bool skip() const { return cs && ir_inner_open(cs); }
   __device__ __forceinline__ void row(uint32_t i, double v, double (&acc)[2]) const {
     const double t = b[i] - v;
     const float rh = (float)(sc[i] * t);
@@ -720,7 +729,8 @@ struct FinIrInit {
   double orelTol, oabsTol;  // outer
   uint32_t maxIter;
   int first;
-  __device__ bool skip() const { return false; }
+  const IrState* cs;  // see EpiResidScaled
+  __device__ bool skip() const { return cs && ir_inner_open(cs); }
   __device__ void fin(double (&s)[2]) const {
     if (threadIdx.x != 0) return;
     const double resAbs = sqrt(s[0]);
@@ -896,8 +906,11 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
 }
 
 // x += s_i d_i: the scaled correction back in the unscaled unknowns.
+// x += S y; with cs, only after an inner solve that ran at least one
+// iteration to its end (y holds that solve's increment).
 __global__ void k_ir_add_scaled(double* __restrict__ x, const double* __restrict__ sc,
-                                const float* __restrict__ y, uint32_t n) {
+                                const float* __restrict__ y, uint32_t n, const IrState* cs) {
+  if (cs && (ir_inner_open(cs) || cs->numIter == 0)) return;
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
     x[i] = fma(sc[i], (double)y[i], x[i]);
@@ -1390,7 +1403,9 @@ struct IrInner {
   // Iteration hint launches the step that finds the loop stopped, so the
   // loop is predicted to launch hint + 1; the replay predicted to end it is
   // cut to that count (rounded up to even).
-  int run(uint32_t maxIter, IrState& out, uint32_t hint) {
+  // behind / behind_ran: see replay_loop.
+  int run(uint32_t maxIter, IrState& out, uint32_t hint,
+          const std::function<int()>& behind = nullptr, bool* behind_ran = nullptr) {
     ReplayPace pace;
     pace.hint = hint;
     const uint64_t need = hint ? (uint64_t)hint + 1 : 0;
@@ -1399,7 +1414,7 @@ struct IrInner {
         W, st, pace, need, (uint64_t)maxIter / kChunk + 4 + need / 2,
         [&](int c, int slot) { return launch_iteration(c, slot >= 0 ? hslot(slot) : nullptr); },
         [&](const IrState& h) { return pace.more(h.gamma, h.thr, h.numIter, h.gamma0); },  // squared
-        &hs, "IR inner"));
+        &hs, "IR inner", 2, UINT64_MAX, behind, behind_ran));
     out = *hs;
     return 0;
   }
@@ -1663,21 +1678,31 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
   double resAbs = 0.0, denAbs = 1.0;
   int rc = 0, err = 0;
   const fx_params& outer = ir->outer;
+  // Scaled: x += S y after an inner solve, then r = b - A x (f64 values,
+  // Apply + XPAY order), W.r = (float)(S r), ||r||, the outer decision and
+  // the next inner solve's starting state.  cs: see EpiResidScaled.
+  auto outer_step = [&](bool add, const IrState* cs) -> int {
+    if (add) {
+      hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
+                         (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x, n, cs);
+      FX_HIP(hipGetLastError());
+    }
+    FX_TRY((launch_spmv<double, double, ROW_AX, 2, 0u>(
+        A, st, XLoad<double>{(const double*)x->d}, BNone{},
+        EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc, cs}, ir->part, nullptr,
+        nullptr, false)));
+    return launch_finish<2, 0u>(
+        st, ir->part, spmv_grid_of<double>(A, ctx->sms),
+        FinIrInit{ir->hrec, ir->dsc + 1, (IrState*)ir->W->state, ir->inner.relTol,
+                  ir->inner.absTol, outer.relTol, outer.absTol, ir->inner.maxIter,
+                  outer_it == 0 ? 1 : 0, cs});
+  };
+  bool step_queued = false;  // the next residual is already queued
   try {
     for (;;) {
-      // r = b - A x (f64 values, Apply + XPAY order), r32 = (float) r, ||r||
-      // (scaled: W.r = (float)(S r), and the inner solve's starting state)
       if (ir->scaled) {
-        rc = launch_spmv<double, double, ROW_AX, 2, 0u>(
-            A, st, XLoad<double>{(const double*)x->d}, BNone{},
-            EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc}, ir->part, nullptr,
-            nullptr, false);
-        if (!rc)
-          rc = launch_finish<2, 0u>(
-              st, ir->part, spmv_grid_of<double>(A, ctx->sms),
-              FinIrInit{ir->hrec, ir->dsc + 1, (IrState*)ir->W->state, ir->inner.relTol,
-                        ir->inner.absTol, outer.relTol, outer.absTol, ir->inner.maxIter,
-                        outer_it == 0 ? 1 : 0});
+        if (!step_queued) rc = outer_step(false, nullptr);
+        step_queued = false;
         if (rc) break;
         // When the last solve went on past this residual, the outer loop is
         // predicted to go on: the inner solve is queued at once, without
@@ -1724,7 +1749,13 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         // the prediction: this outer step's inner count in the last solve
         const size_t step = outer_it - 1;
         const uint32_t hint = step < ir->inner_counts.size() ? ir->inner_counts[step] : 0u;
-        rc = in.run(ir->inner.maxIter, fs, hint);
+        // the next outer step is queued right behind the inner solve's
+        // predicted last replay, to run only if the inner solve has ended
+        // by then
+        bool behind_ran = false;
+        rc = in.run(ir->inner.maxIter, fs, hint,
+                    [&]() { return outer_step(true, (const IrState*)ir->W->state); },
+                    &behind_ran);
         if (rc) break;
         if (fs.reason == IR_OUTER) {  // queued ahead, but the outer loop was done
           --outer_it;
@@ -1733,13 +1764,12 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
         if (ir->inner_counts.size() <= step) ir->inner_counts.resize(step + 1, 0u);
         ir->inner_counts[step] = fs.reason == IR_CONVERGED ? fs.numIter : 0;
         inner_total += fs.numIter;
-        if (fs.numIter > 0) {
-          hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
-                             (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x, n);
-          if (hipGetLastError() != hipSuccess) {
-            rc = FX_ERROR_DEVICE;
-            break;
-          }
+        if (behind_ran) {
+          step_queued = true;
+        } else {
+          rc = outer_step(fs.numIter > 0, nullptr);
+          if (rc) break;
+          step_queued = true;
         }
         continue;
       }

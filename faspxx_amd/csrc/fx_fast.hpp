@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cmath>
+#include <functional>
 
 #include "fx_solver.hpp"
 #include "fx_spmv.hpp"
@@ -241,10 +242,14 @@ int replay_launch(FastWs& W, hipStream_t st, int slot, int L, Iter& iter) {
 // unit: see replay_len.  max_launch: iterations after which the loop has
 // certainly stopped; no replay is queued beyond them while one is in
 // flight.  At most max_chunks replays may find the loop running.
+// behind: work enqueued right behind the replay predicted to end the loop
+// (none when null); *behind_ran is then whether that replay or an earlier
+// one ended it, i.e. whether the work found the loop stopped.
 template <class State, class Iter, class More>
 int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need,
                 uint64_t max_chunks, Iter&& iter, More&& more, const State** out,
-                const char* what, int unit = 2, uint64_t max_launch = UINT64_MAX) {
+                const char* what, int unit = 2, uint64_t max_launch = UINT64_MAX,
+                const std::function<int()>& behind = nullptr, bool* behind_ran = nullptr) {
   const int full = replay_len(0, 0, unit);
   if (st) {  // the graphs this loop may replay, made before any replay is queued
     const int tail = replay_len(need, need / full * full, unit);
@@ -255,13 +260,18 @@ int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need
     }
   }
   SlotQueue sq;
-  uint64_t queued = 0, chunks = 0;
+  uint64_t queued = 0, chunks = 0, synced = 0, behind_at = 0;
+  if (behind_ran) *behind_ran = false;
   auto queue = [&]() -> int {
     const int slot = sq.free_slot(), L = replay_len(need, queued, unit);
     FX_TRY(replay_launch(W, st, slot, L, iter));
     sq.push(slot);
     queued += (uint64_t)L;
     ++chunks;
+    if (behind && !behind_at && need && queued >= need) {
+      FX_TRY(behind());
+      behind_at = chunks;
+    }
     return 0;
   };
   FX_TRY(queue());
@@ -270,9 +280,11 @@ int replay_loop(FastWs& W, hipStream_t st, const ReplayPace& pace, uint64_t need
     const int cur = sq.front();
     FX_HIP(hipEventSynchronize(W.ev[cur]));
     sq.pop();
+    ++synced;
     const State* hs = (const State*)W.hstate + cur;
     if (hs->status != ST_RUN) {
       *out = hs;
+      if (behind_ran) *behind_ran = behind_at && synced <= behind_at;
       return 0;
     }
     FX_CHECK(chunks <= max_chunks, FX_ERROR_DEVICE, "%s device loop did not stop", what);
