@@ -716,6 +716,34 @@ struct EpiResidScaled {
   __device__ void fin(double (&)[2]) const {}
 };
 
+// EpiResidScaled for a matrix stored with its rows sorted by length: b and
+// S come in that stored order (bo[pos] = b[row]), so that they are read in
+// whole lines; the same values as EpiResidScaled's.
+struct EpiResidScaledSorted {
+  float* __restrict__ r32;
+  const double* __restrict__ bo;
+  const double* __restrict__ sco;
+  const IrState* cs;
+  __device__ bool skip() const { return cs && ir_inner_open(cs); }
+  __device__ __forceinline__ void row_at(uint32_t pos, uint32_t i, double v,
+                                         double (&acc)[2]) const {
+    const double t = bo[pos] - v;
+    const float rh = (float)(sco[pos] * t);
+    r32[i] = rh;
+    acc[0] = fma(t, t, acc[0]);
+    acc[1] = fma((double)rh, (double)rh, acc[1]);
+  }
+  __device__ __forceinline__ void row(uint32_t, double, double (&)[2]) const {}
+  __device__ void fin(double (&)[2]) const {}
+};
+
+// dst[pos] = src[omap[pos]]: a vector in a sorted matrix's stored row order.
+__global__ void k_gather_rows(const uint32_t* __restrict__ omap, const double* __restrict__ src,
+                              double* __restrict__ dst, uint32_t n) {
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) dst[i] = src[omap[i]];
+}
+
 // The outer loop's record (||t||, its test against the outer tolerance; the
 // first residual of a solve sets the reference value), and the inner
 // solve's starting state: running, or stopped (IR_OUTER) when the outer
@@ -1521,6 +1549,8 @@ struct fx_ir_s {
   std::vector<uint32_t> inner_counts;  // each outer step's inner iterations, last solve
   uint32_t outer_last = 0;             // outer steps of the last solve (0: none)
   IrOuterRec* hrec = nullptr;          // pinned
+  double* sco = nullptr;  // sorted matrices: S and b in the stored row order
+  double* bo = nullptr;
   int* bad = nullptr;
   uint32_t* blk512 = nullptr;  // row-pattern matrices: As's own 512-row blocks
   uint16_t* rpb512 = nullptr;
@@ -1536,6 +1566,8 @@ struct fx_ir_s {
     if (hrec) (void)hipHostFree(hrec);
     (void)hipFree(part);
     (void)hipFree(sc);
+    (void)hipFree(sco);
+    (void)hipFree(bo);
     (void)hipFree(vs32);
     (void)hipFree(vs32o);
     (void)hipFree(bad);
@@ -1585,6 +1617,12 @@ int ir_scale(fx_ir ir) {
     FX_HIP(hipMalloc(&ir->vs32o, sizeof(float) * (A->nnz ? A->nnz : 1)));
     FX_TRY(sorted_fill_f32(A, ir->vs32, ir->vs32o));
     ir->As.oval32 = ir->vs32o;
+    // S, and room for b, in the stored row order for the f64 residual
+    FX_HIP(hipMalloc(&ir->sco, sizeof(double) * pack_round(n)));
+    FX_HIP(hipMalloc(&ir->bo, sizeof(double) * pack_round(n)));
+    hipLaunchKernelGGL(k_gather_rows, dim3(stream_blocks(n)), dim3(kThreads), 0, st, A->omap,
+                       (const double*)ir->sc, ir->sco, n);
+    FX_HIP(hipGetLastError());
   }
   if (A->pid) {
     // the f32 inner SpMV of a row-pattern matrix runs on 512-row blocks:
@@ -1687,10 +1725,16 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
                          (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x, n, cs);
       FX_HIP(hipGetLastError());
     }
-    FX_TRY((launch_spmv<double, double, ROW_AX, 2, 0u>(
-        A, st, XLoad<double>{(const double*)x->d}, BNone{},
-        EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc, cs}, ir->part, nullptr,
-        nullptr, false)));
+    if (ir->bo)
+      FX_TRY((launch_spmv<double, double, ROW_AX, 2, 0u>(
+          A, st, XLoad<double>{(const double*)x->d}, BNone{},
+          EpiResidScaledSorted{(float*)ir->W->r, ir->bo, ir->sco, cs}, ir->part, nullptr,
+          nullptr, false)));
+    else
+      FX_TRY((launch_spmv<double, double, ROW_AX, 2, 0u>(
+          A, st, XLoad<double>{(const double*)x->d}, BNone{},
+          EpiResidScaled{(float*)ir->W->r, (const double*)b->d, ir->sc, cs}, ir->part, nullptr,
+          nullptr, false)));
     return launch_finish<2, 0u>(
         st, ir->part, spmv_grid_of<double>(A, ctx->sms),
         FinIrInit{ir->hrec, ir->dsc + 1, (IrState*)ir->W->state, ir->inner.relTol,
@@ -1698,8 +1742,13 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
                   outer_it == 0 ? 1 : 0, cs});
   };
   bool step_queued = false;  // the next residual is already queued
+  if (ir->scaled && ir->bo) {  // b in the sorted matrix's stored row order
+    hipLaunchKernelGGL(k_gather_rows, dim3(stream_blocks(n)), dim3(kThreads), 0, st, A->omap,
+                       (const double*)b->d, ir->bo, n);
+    if (hipGetLastError() != hipSuccess) rc = FX_ERROR_DEVICE;
+  }
   try {
-    for (;;) {
+    while (!rc) {
       if (ir->scaled) {
         if (!step_queued) rc = outer_step(false, nullptr);
         step_queued = false;

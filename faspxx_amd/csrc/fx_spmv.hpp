@@ -557,6 +557,21 @@ __device__ __forceinline__ void sell_block(const SpmvIn<MT, VT>& in, uint32_t w,
   if ((uint32_t)tid < nr) epi.row(r0 + (uint32_t)tid, rsum[tid], acc);
 }
 
+// An epilogue may take the row's position in the matrix's stored row order
+// as well (row_at(pos, row, v, acc)), for a matrix whose rows are stored
+// sorted by length: vectors kept in that order are then read in whole lines.
+template <class T, class A, class = void>
+struct has_row_at : std::false_type {};
+template <class T, class A>
+struct has_row_at<T, A, std::void_t<decltype(std::declval<const T&>().row_at(
+                             0u, 0u, 0.0, std::declval<A&>()))>> : std::true_type {};
+template <class Epi, class A>
+__device__ __forceinline__ void emit_row(const Epi& epi, uint32_t pos, uint32_t row, double v,
+                                         A& acc) {
+  if constexpr (has_row_at<Epi, A>::value) epi.row_at(pos, row, v, acc);
+  else epi.row(row, v, acc);
+}
+
 // Generic kernel.  X is a functor giving the gathered vector entry x_j
 // (a plain load, or a value formed on the fly); B gives b_i for
 // ROW_B_MINUS_AX.  Epi must provide:
@@ -617,7 +632,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
       if (tid == 0) {
         const uint32_t orr = orow(r0);
         const double v = (MODE == ROW_AX) ? t[0] : (double)bf(orr) - t[0];
-        epi.row(orr, v, acc);
+        emit_row(epi, r0, orr, v, acc);
       }
     } else if constexpr (is_pat(CF)) {
       in.va ? pat_block<CF, true, MODE, NRED>(in, lds, r0, k0, nr, nz, tid, xf, bf, epi, acc)
@@ -673,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
               if (tid == 0) {
                 const uint32_t orr = orow(r0 + rr);
                 const double v = (MODE == ROW_AX) ? tt[0] : (double)bf(orr) - tt[0];
-                epi.row(orr, v, acc);
+                emit_row(epi, r0 + rr, orr, v, acc);
               }
             }
           }
@@ -708,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
               }
             }
           }
-          epi.row(orr, t, acc);
+          emit_row(epi, r0 + row, orr, t, acc);
         }
       } else {
         const int grp = tid / tpr, lane = tid % tpr, ngrp = kThreads / tpr;
@@ -735,7 +750,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(SpmvIn<MT, VT> in, XF xf0, BF
           if (lane == 0) {
             const uint32_t orr = orow(r0 + rr);
             const double v = (MODE == ROW_AX) ? t : (double)bf(orr) - t;
-            epi.row(orr, v, acc);
+            emit_row(epi, r0 + rr, orr, v, acc);
           }
         }
       }

@@ -697,3 +697,27 @@ def test_ir_outer_steps_queued_ahead(ctx, problems):
     for k in (2, 3):
         assert runs[k][:2] == runs[0][:2], (runs[k][:2], runs[0][:2])
         assert np.array_equal(runs[k][2], runs[0][2])
+
+
+def test_ir_sorted_rows(ctx, problems):
+    """The refinement on a matrix stored with its rows sorted by length (the
+    power-law system at n = 200000): its f64 residual reads b and the
+    scaling in that stored order.  Two solves on one object (the second
+    with its outer steps queued ahead and behind) reach the outer tolerance
+    in true FP64 residual and agree bit for bit."""
+    pr = problems(4, 200000)
+    n = pr.A.nrow
+    A = fx.Csr.from_problem(ctx, pr.A)
+    assert A.layout == "sorted"
+    ir = fx.IR(ctx, A, 1.0, fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
+               fx.params(maxIter=5000, relTol=1e-4, absTol=0.0))
+    runs = []
+    for _ in range(2):
+        x = fx.Vec(ctx, n)
+        res, inner = ir.Solve(dev(ctx, pr.b), x)
+        assert res.retcode == 0
+        runs.append((res.numIter, inner, x.numpy()))
+    ir.close()
+    xr = runs[0][2]
+    assert np.linalg.norm(pr.b - pr.A.matvec(xr)) < 1e-10 * np.linalg.norm(pr.b)
+    assert runs[1][:2] == runs[0][:2] and np.array_equal(runs[1][2], xr)
