@@ -175,6 +175,15 @@ struct CgSide {  // written by K2's workgroup 0 for K3
   int breakdown, pad;
 };
 
+// Workgroups per CU of the CG's K2 and K3: 1 measured best for the FP64 CG
+// on the 128^3 stencil (19.7-19.9 ms; 2: 20.9-21.2 ms; with 2 packs per
+// pass, 20.3 ms and at 4 per CU 23.5 ms): each workgroup re-sums all of the
+// SpMV's 8192 partials.
+#ifndef FX_CG_PER_CU
+#define FX_CG_PER_CU 1
+#endif
+constexpr int kCgPerCu = FX_CG_PER_CU;
+
 // One pass of K2 over packs c + i * stride, i < kPassPacks (all loaded
 // before use; packs at or past nc are clamped to pack 0, which the padded
 // vectors hold, and their results dropped).
@@ -794,18 +803,31 @@ struct EpiIrW {
   __device__ void fin(double (&)[1]) const {}
 };
 
-// One pass of K2 over packs c + i * stride, i < kPassPacks (all loaded
+// Packs per thread per pass of the refinement's K2, and its workgroups per
+// CU: 2 and 2 (512 workgroups of one pass each on the 128^3 stencil)
+// measured 9.90-9.94 ms for the refinement against 10.20-10.26 ms for 4 and
+// 1 (256 workgroups); 2 passes of 2 packs, or 1 pack, were slower.
+#ifndef FX_IR_PACKS
+#define FX_IR_PACKS 2
+#endif
+#ifndef FX_IR_PER_CU
+#define FX_IR_PER_CU 2
+#endif
+constexpr int kIrPacks = FX_IR_PACKS;
+constexpr int kIrPerCu = FX_IR_PER_CU;
+
+// One pass of K2 over packs c + i * stride, i < kIrPacks (all loaded
 // before use; packs at or past nc are clamped to pack 0 and dropped).
 struct IrPass {
-  Pack16<float> r[kPassPacks], w[kPassPacks], p[kPassPacks], s[kPassPacks], y[kPassPacks];
-  uint32_t c[kPassPacks];
-  bool ok[kPassPacks];
+  Pack16<float> r[kIrPacks], w[kIrPacks], p[kIrPacks], s[kIrPacks], y[kIrPacks];
+  uint32_t c[kIrPacks];
+  bool ok[kIrPacks];
   __device__ __forceinline__ void load(const float* __restrict__ rv, const float* __restrict__ wv,
                                        const float* __restrict__ pv, const float* __restrict__ sv,
                                        const float* __restrict__ yv, uint32_t c0, uint32_t stride,
                                        uint32_t nc) {
 #pragma unroll
-    for (int i = 0; i < kPassPacks; ++i) {
+    for (int i = 0; i < kIrPacks; ++i) {
       const uint32_t ci = c0 + i * stride;
       ok[i] = ci < nc;
       c[i] = ok[i] ? ci : 0u;
@@ -893,7 +915,7 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
   double acc = 0.0;
   for (uint32_t c = cfirst;;) {
 #pragma unroll
-    for (int i = 0; i < kPassPacks; ++i) {
+    for (int i = 0; i < kIrPacks; ++i) {
       double ai = 0.0;
 #pragma unroll
       for (int k = 0; k < W; ++k)
@@ -907,7 +929,7 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
         acc += ai;
       }
     }
-    c += kPassPacks * stride;
+    c += kIrPacks * stride;
     if (c >= nc) break;
     pass.load(r, w, p, s, y, c, stride, nc);
   }
@@ -1017,7 +1039,7 @@ struct CgRun {
     FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{p}, BNone{},
                                           EpiCgQ<VT>{q, p, dstate(c)}, part1())));
     const unsigned g1 = fused_grid<VT>(A, op, n, cus);
-    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks, kCgPerCu);
     hipLaunchKernelGGL((k_cg_update_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, x, r, p, q,
                        dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
                        (CgSide*)W.side, part2());
@@ -1043,7 +1065,7 @@ struct CgRun {
     FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{p}, BNone{},
                                           EpiCgQ<VT>{q, p, dstate(c)}, part1())));
     const unsigned g1 = fused_grid<VT>(A, op, n, cus);
-    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kPassPacks, kCgPerCu);
     hipLaunchKernelGGL((k_cg_update_w<VT, true, true>), dim3(g), dim3(kWideThreads), 0, st, x, r,
                        p, q, dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
                        (CgSide*)W.side, part2());
@@ -1416,7 +1438,7 @@ struct IrInner {
                                                      EpiIrW{w, r, dstate(c)}, W.partials,
                                                      nullptr, nullptr, false)));
     const unsigned g1 = spmv_grid_of<float>(As, W.ctx->sms);
-    const unsigned g = wide_blocks(W.n, Pack16<float>::W, W.ctx->sms, kPassPacks);
+    const unsigned g = wide_blocks(W.n, Pack16<float>::W, W.ctx->sms, kIrPacks, kIrPerCu);
     hipLaunchKernelGGL(k_ir_update, dim3(g), dim3(kWideThreads), 0, st, r, (const float*)w,
                        (float*)W.p, (float*)W.x2, (float*)W.x, W.n, dstate(c), dstate(c + 1),
                        hout, (const double*)W.partials, g1, (const double*)pg(c), pg(c + 1));
