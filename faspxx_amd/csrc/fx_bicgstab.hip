@@ -38,6 +38,14 @@ struct BiState {
   int status, reason, err, pad;
 };
 
+// Workgroups per CU of the streaming kernels B2, B4 and B5: 1 measured best
+// on the 1024^2 upwind system (63.4 us per iteration; 2: 70.4 us, with a
+// different count, since the partial sums then round differently).
+#ifndef FX_BI_PER_CU
+#define FX_BI_PER_CU 1
+#endif
+constexpr int kBiPerCu = FX_BI_PER_CU;
+
 struct BiSide {  // scalars handed from B2 / B4 to the later kernels of an iteration
   double t12, alpha, omega;
   int stop, pad;
@@ -487,7 +495,7 @@ struct BiRun {
     const uint32_t n = W.n;
     const int cus = W.ctx->sms;
     const unsigned gs = fused_grid<VT>(A, op, n, cus);
-    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, 2, kBiPerCu);
     BiSide* side = (BiSide*)W.side;
     FX_TRY((fused_spmv<VT, ROW_AX, 1, 0u>(A, op, W, st, XLoad<VT>{vp()}, BNone{},
                                           EpiBiP<VT, JAC>{vpt(), dinv, vr0s(), dstate(c)},

@@ -183,6 +183,13 @@ struct CgSide {  // written by K2's workgroup 0 for K3
 #define FX_CG_PER_CU 1
 #endif
 constexpr int kCgPerCu = FX_CG_PER_CU;
+// Workgroups per CU of K3 alone (it reads K2's partials, whatever its grid):
+// 1 measured best (FP64 CG on the 128^3 stencil 19.6-19.9 ms; 2: 20.4 ms;
+// 4: 22.4 ms).
+#ifndef FX_K3_PER_CU
+#define FX_K3_PER_CU 1
+#endif
+constexpr int kK3PerCu = FX_K3_PER_CU;
 
 // One pass of K2 over packs c + i * stride, i < kPassPacks (all loaded
 // before use; packs at or past nc are clamped to pack 0, which the padded
@@ -1044,7 +1051,8 @@ struct CgRun {
                        dinv, n, (const CgState*)dstate(c), (const double*)part1(), g1,
                        (CgSide*)W.side, part2());
     FX_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_cg_pupdate_w<VT, JAC>), dim3(g), dim3(kWideThreads), 0, st, p, r, dinv,
+    const unsigned g3 = wide_blocks(n, Pack16<VT>::W, cus, kPPacks, kK3PerCu);
+    hipLaunchKernelGGL((k_cg_pupdate_w<VT, JAC>), dim3(g3), dim3(kWideThreads), 0, st, p, r, dinv,
                        n, dstate(c), dstate(c + 1), hout, (const double*)part2(), g,
                        (const CgSide*)W.side, P, W.hist);
     FX_HIP(hipGetLastError());
@@ -1100,7 +1108,8 @@ struct CgRun {
             part1(), nullptr, nullptr, false)));
       zc = zn;
     }
-    hipLaunchKernelGGL((k_cg_pupdate_w<VT, true, true>), dim3(g), dim3(kWideThreads), 0, st, p,
+    const unsigned g3 = wide_blocks(n, Pack16<VT>::W, cus, kPPacks, kK3PerCu);
+    hipLaunchKernelGGL((k_cg_pupdate_w<VT, true, true>), dim3(g3), dim3(kWideThreads), 0, st, p,
                        zc, dinv, n, dstate(c), dstate(c + 1), hout, (const double*)part2(), g,
                        (const CgSide*)W.side, P, W.hist, (const double*)part1(), g1);
     FX_HIP(hipGetLastError());
