@@ -41,6 +41,19 @@ namespace {
 
 enum { GM_RUN = 0, GM_CYCLE_END = 1 };
 
+// Grid of the cycle's streaming kernels (MGS steps, Givens, updates): packs
+// per thread, workgroups per CU.  GMRES(30) on the 1024^2 upwind system:
+// 2 and 2 (512 workgroups) 109.8 us per iteration; 1 and 2, or 2 and 4, the
+// same; 1 and 4 (1024 workgroups, each re-summing the partials) 135 us.
+#ifndef FX_GM_PER
+#define FX_GM_PER 2
+#endif
+#ifndef FX_GM_PER_CU
+#define FX_GM_PER_CU 2
+#endif
+constexpr int kGmPer = FX_GM_PER;
+constexpr int kGmPerCu = FX_GM_PER_CU;
+
 struct GmState {
   double resAbs, rj, denAbs, pad0;
   uint32_t numIter, count, status, histCount;
@@ -566,7 +579,7 @@ struct GmRun {
     // two workgroups per CU: the MGS steps (which re-sum only the previous
     // step's few hundred partials) keep twice the loads in flight, GMRES(30)
     // on the 1024^2 upwind system 238 against 250 ms
-    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, 2, 2);
+    const unsigned g = wide_blocks(n, Pack16<VT>::W, cus, kGmPer, kGmPerCu);
     const GmLsq L = lsq();
     const GmPrm P = prm(m);
     double* pa = W.partials;
