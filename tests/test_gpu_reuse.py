@@ -184,3 +184,40 @@ def test_null_stream_context_refinement(nctx, oracle, problems, cid, scale):
     o = oracle.solve(CG, M, pr.b, np.zeros(M.nrow), pc="jacobi", maxIter=20000, relTol=1e-12,
                      absTol=0.0)
     assert rel_err(xr, o["x"]) < 1e-6
+
+
+def test_null_stream_context_repeated(nctx, problems):
+    """Second and third solves on a null-stream context, whose replays are
+    launched directly: the refinement with its outer steps queued ahead of
+    and behind its inner solves, CG and the Jacobi sweeps with their last
+    replay cut to the predicted count.  Each repeat agrees bit for bit with
+    the first solve."""
+    pr = problems(2, 24)
+    n = pr.A.nrow
+    A = fx.Csr.from_problem(nctx, pr.A)
+    ir = fx.IR(nctx, A, 1.0, fx.params(maxIter=50, relTol=1e-10, absTol=0.0),
+               fx.params(maxIter=5000, relTol=1e-4, absTol=0.0))
+    runs = []
+    for _ in range(3):
+        x = fx.Vec(nctx, n)
+        res, inner = ir.Solve(dev(nctx, pr.b), x)
+        assert res.retcode == 0
+        runs.append((res.numIter, inner, x.numpy()))
+    ir.close()
+    for k in (1, 2):
+        assert runs[k][:2] == runs[0][:2] and np.array_equal(runs[k][2], runs[0][2])
+    for kind, pc_w, prm in ((fx.CG, 1.0, dict(maxIter=2000, relTol=1e-8, absTol=0.0)),
+                            (fx.JACOBI, 2.0 / 3.0, dict(maxIter=40, relTol=1e-30, absTol=0.0))):
+        P = fx.Precond(nctx, "jacobi", A, pc_w, 1)
+        s = fx.Solver(nctx, kind)
+        s.Setup(A, P, fx.params(**prm))
+        outs = []
+        for _ in range(3):
+            x = dev(nctx, np.zeros(n))
+            res = s.Solve(dev(nctx, pr.b), x)
+            assert res.retcode == 0
+            outs.append((res.numIter, x.numpy()))
+        for k in (1, 2):
+            assert outs[k][0] == outs[0][0] and np.array_equal(outs[k][1], outs[0][1])
+        s.close()
+        P.close()
