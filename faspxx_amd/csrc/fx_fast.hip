@@ -513,7 +513,15 @@ struct EpiJacobiLag {
   uint32_t buf_prev, buf_cur;  // the x buffers of x_{m-1} and x_m
   double* hist;
   int use_v;  // ||x_m||_inf >= 1e-20: work = b - A x, else work = b
-  __device__ bool skip() const { return s_in->status != ST_RUN; }
+  // Stopped; or, outside workgroup 0, the sweep that decides sweep
+  // maxIter - 1: that decision ends the solve whatever it finds, so the
+  // sweep's own x_{m+1} and partial sums are never read and only workgroup
+  // 0 (the decision) runs.  Workgroup 0 may be writing s_in meanwhile; every
+  // mix of the old and new fields it can leave reads as skip here.
+  __device__ bool skip() const {
+    if (s_in->status != ST_RUN) return true;
+    return blockIdx.x != 0 && s_in->pending && s_in->numIter + 1 >= P.maxIter;
+  }
   __device__ void init() { use_v = *flag_in; }
   __device__ void block_start() {
     if (blockIdx.x != 0) return;
@@ -1399,30 +1407,37 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b_user, double* x_user, fx_
   FX_TRY(replay_loop<JacLagState>(
       W, st, pace, need, max_launch / kJacPeriod + 4, sweep,
       [](const JacLagState&) { return true; }, &hs, "Jacobi", kJacPeriod, max_launch));
-  JacLagState s{};
-  FX_HIP(hipStreamSynchronize(st));
-  FX_HIP(hipMemcpy(&s, ds, sizeof(JacLagState), hipMemcpyDeviceToHost));
+  // the state the loop stopped on, from the pinned slot of the replay that
+  // reported it: the stopping sweep writes it to both state buffers, and
+  // the sweeps after it skip, so it is final
+  const JacLagState s = *hs;
   FX_CHECK(s.answer < 3, FX_ERROR_DEVICE, "Jacobi answer buffer %u", s.answer);
   double* ans = xbuf[s.answer];
   res->numIter = s.numIter;
   W.last_iters = s.numIter;
-  if (!(s.numIter == PP.minIter && s.numIter == PP.maxIter)) {
+  // the final residual, the answer and the history, queued behind the loop
+  // with one wait for all of them
+  const bool norms = !(s.numIter == PP.minIter && s.numIter == PP.maxIter);
+  if (norms) {
     FX_TRY(norminf_to(st, ans, n, W.dscal, W.partials));
     FX_TRY((launch_spmv<double, double, ROW_B_MINUS_AX, 2, 2u>(
         A, st, XLoad<double>{ans}, BLoad<double>{b}, EpiResNorms{b, W.dscal, W.dscal + 1},
         W.partials)));
-    double h[3];
-    FX_HIP(hipMemcpyAsync(h, W.dscal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
-    FX_HIP(hipStreamSynchronize(st));
+  }
+  FX_HIP(hipMemcpyAsync(x_user, ans, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  double h[3] = {0.0, 0.0, 0.0};
+  if (norms) FX_HIP(hipMemcpyAsync(h, W.dscal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+  const uint32_t hc = s.histCount < W.hist_cap ? s.histCount : W.hist_cap;
+  S->hist_host.assign(hc, 0.0);
+  if (hc)
+    FX_HIP(hipMemcpyAsync(S->hist_host.data(), W.hist, sizeof(double) * hc,
+                          hipMemcpyDeviceToHost, st));
+  S->hist_count = hc;
+  FX_HIP(hipStreamSynchronize(st));
+  if (norms) {
     res->norm2 = h[1];
     res->normInf = h[2];
   }
-  FX_HIP(hipMemcpyAsync(x_user, ans, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-  const uint32_t hc = s.histCount < W.hist_cap ? s.histCount : W.hist_cap;
-  S->hist_host.assign(hc, 0.0);
-  if (hc) FX_HIP(hipMemcpy(S->hist_host.data(), W.hist, sizeof(double) * hc, hipMemcpyDeviceToHost));
-  S->hist_count = hc;
-  FX_HIP(hipStreamSynchronize(st));
   return FX_SUCCESS;
 }
 
