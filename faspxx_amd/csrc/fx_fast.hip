@@ -837,10 +837,11 @@ struct IrPass {
   Pack16<float> r[kIrPacks], w[kIrPacks], p[kIrPacks], s[kIrPacks], y[kIrPacks];
   uint32_t c[kIrPacks];
   bool ok[kIrPacks];
+  // ld_y: y is read only by the iterations that update it (see ir_elem)
   __device__ __forceinline__ void load(const float* __restrict__ rv, const float* __restrict__ wv,
                                        const float* __restrict__ pv, const float* __restrict__ sv,
                                        const float* __restrict__ yv, uint32_t c0, uint32_t stride,
-                                       uint32_t nc) {
+                                       uint32_t nc, bool ld_y) {
 #pragma unroll
     for (int i = 0; i < kIrPacks; ++i) {
       const uint32_t ci = c0 + i * stride;
@@ -850,18 +851,27 @@ struct IrPass {
       w[i] = ld16(wv, c[i]);
       p[i] = ld16(pv, c[i]);
       s[i] = ld16(sv, c[i]);
-      y[i] = ld16(yv, c[i]);
+      if (ld_y) y[i] = ld16(yv, c[i]);
     }
   }
 };
 
-// The updates of one element; first: the iteration that starts the
-// recurrences (p = r, s = w, y = alpha p; the old p, s, y are not used).
-__device__ __forceinline__ void ir_elem(bool first, float a, float b, float& ri, float wi,
-                                        float& pi, float& si, float& yi, double& acc) {
+// The updates of one element in iteration it; first (it == 0): the
+// iteration that starts the recurrences (p = r, s = w; the old p and s are
+// not used).  y += alpha p is applied two iterations at a time, so that y
+// is read and written every other iteration: an odd iteration (yup) forms
+// y = fma(a, p, fma(ap, p_old, y)) from the previous iteration's alpha (ap)
+// and p, which are the same two roundings, in the same order, as updating y
+// in each; at it == 1 (ypair0) the first term is ap * p_old, as iteration
+// 0's y = alpha p.  After an odd number of iterations the last one's term
+// is still to be added (k_ir_add_scaled does it).
+__device__ __forceinline__ void ir_elem(bool first, bool yup, bool ypair0, float a, float b,
+                                        float ap, float& ri, float wi, float& pi, float& si,
+                                        float& yi, double& acc) {
+  const float po = pi;
   pi = first ? ri : fma(b, pi, ri);
   si = first ? wi : fma(b, si, wi);
-  yi = first ? a * pi : fma(a, pi, yi);
+  if (yup) yi = fma(a, pi, ypair0 ? ap * po : fma(ap, po, yi));
   ri = fma(-a, si, ri);
   acc = fma((double)ri, (double)ri, acc);
 }
@@ -882,9 +892,10 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
   constexpr int W = Pack16<float>::W;
   const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
   const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
-  IrPass pass;  // the first pass is loaded before the scalars are known
-  pass.load(r, w, p, s, y, cfirst, stride, nc);
   const uint32_t it = st_cur->numIter;
+  const bool yup = (it & 1u) != 0;  // this iteration updates y (for it - 1 and it)
+  IrPass pass;  // the first pass is loaded before the scalars are known
+  pass.load(r, w, p, s, y, cfirst, stride, nc, yup);
   double red[2];
   {
     double d1[1], d2[1] = {0.0};
@@ -925,8 +936,9 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
     }
     return;
   }
-  const bool first = it == 0;
+  const bool first = it == 0, ypair0 = it == 1;
   const float a = (float)alpha, b = (float)beta;
+  const float ap = (float)st_cur->alpha_prev;  // iteration it - 1's (read when yup)
   double acc = 0.0;
   for (uint32_t c = cfirst;;) {
 #pragma unroll
@@ -934,28 +946,28 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
       double ai = 0.0;
 #pragma unroll
       for (int k = 0; k < W; ++k)
-        ir_elem(first, a, b, pass.r[i].v[k], pass.w[i].v[k], pass.p[i].v[k], pass.s[i].v[k],
-                pass.y[i].v[k], ai);
+        ir_elem(first, yup, ypair0, a, b, ap, pass.r[i].v[k], pass.w[i].v[k], pass.p[i].v[k],
+                pass.s[i].v[k], pass.y[i].v[k], ai);
       if (pass.ok[i]) {
         st16(r, pass.c[i], pass.r[i]);
         st16(p, pass.c[i], pass.p[i]);
         st16(s, pass.c[i], pass.s[i]);
-        st16(y, pass.c[i], pass.y[i]);
+        if (yup) st16(y, pass.c[i], pass.y[i]);
         acc += ai;
       }
     }
     c += kIrPacks * stride;
     if (c >= nc) break;
-    pass.load(r, w, p, s, y, c, stride, nc);
+    pass.load(r, w, p, s, y, c, stride, nc, yup);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
     const uint32_t i = nc * W + threadIdx.x;
-    float ri = r[i], pi = p[i], si = s[i], yi = y[i];
-    ir_elem(first, a, b, ri, w[i], pi, si, yi, acc);
+    float ri = r[i], pi = p[i], si = s[i], yi = yup ? y[i] : 0.0f;
+    ir_elem(first, yup, ypair0, a, b, ap, ri, w[i], pi, si, yi, acc);
     r[i] = ri;
     p[i] = pi;
     s[i] = si;
-    y[i] = yi;
+    if (yup) y[i] = yi;
   }
   double v[1] = {acc};
   block_partials_n<1, 0u, kWideThreads>(v, pg_out);
@@ -972,13 +984,24 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
 
 // x += s_i d_i: the scaled correction back in the unscaled unknowns.
 // x += S y; with cs, only after an inner solve that ran at least one
-// iteration to its end (y holds that solve's increment).
+// iteration to its end (y holds that solve's increment).  fin: the inner
+// solve's final state; after an odd number m of iterations y still lacks
+// the last one's alpha p (k_ir_update adds two iterations' terms at a
+// time), which is added here with the same rounding (alpha p alone when
+// m == 1).
 __global__ void k_ir_add_scaled(double* __restrict__ x, const double* __restrict__ sc,
-                                const float* __restrict__ y, uint32_t n, const IrState* cs) {
+                                const float* __restrict__ y, const float* __restrict__ p,
+                                uint32_t n, const IrState* cs, const IrState* fin) {
   if (cs && (ir_inner_open(cs) || cs->numIter == 0)) return;
+  const uint32_t m = fin->numIter;
+  const bool pend = (m & 1u) != 0;
+  const float ap = (float)fin->alpha_prev;
   const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
-    x[i] = fma(sc[i], (double)y[i], x[i]);
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    float yi = m == 1 ? 0.0f : y[i];
+    if (pend) yi = m == 1 ? ap * p[i] : fma(ap, p[i], yi);
+    x[i] = fma(sc[i], (double)yi, x[i]);
+  }
 }
 
 }  // namespace
@@ -1768,7 +1791,8 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
   auto outer_step = [&](bool add, const IrState* cs) -> int {
     if (add) {
       hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
-                         (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x, n, cs);
+                         (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x,
+                         (const float*)ir->W->p, n, cs, (const IrState*)ir->W->state);
       FX_HIP(hipGetLastError());
     }
     if (ir->bo)
