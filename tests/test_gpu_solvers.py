@@ -499,6 +499,98 @@ def test_ir_scaled_inner_iteration_cap(ctx, problems, monkeypatch):
     assert np.linalg.norm(pr.b - pr.A.matvec(xs)) < 1e-10 * np.linalg.norm(pr.b)
 
 
+def _f32_fma(a, b, c):
+    """fma in f32 from f64: the product of two f32 values is exact in f64;
+    the f64 sum is then rounded to f32 (a double rounding, which can differ
+    from one correctly rounded fma by an ulp in rare cases)."""
+    return (np.float64(a) * np.asarray(b, np.float64)
+            + np.asarray(c, np.float64)).astype(np.float32)
+
+
+def _scaled_inner_cg(A, b, m):
+    """m iterations of the refinement's inner solve from x0 = 0, restated in
+    numpy: single-reduction CG on S A S (S = diag(a_ii)^-1/2, values rounded
+    to f32), f32 vectors, f64 sums, y += alpha p in every iteration.  Returns
+    the outer step's x = S y (f64) and the sums (r, r) at the start of
+    iterations 0..m, which decide where a tolerance stops the solve."""
+    M = A.to_scipy().tocsr()
+    d = M.diagonal()
+    s = 1.0 / np.sqrt(d)
+    rows = np.repeat(np.arange(A.nrow), np.diff(A.row_ptr.astype(np.int64)))
+    cols = A.col_ind.astype(np.int64)
+    vs = ((A.values * s[rows]) * s[cols]).astype(np.float32)
+    import scipy.sparse as sp
+    Ah = sp.csr_matrix((vs.astype(np.float64), cols, A.row_ptr.astype(np.int64)),
+                       shape=(A.nrow, A.nrow))
+    r = (s * b).astype(np.float32)
+    gamma0 = float(np.dot(r.astype(np.float64), r.astype(np.float64)))
+    p = sp_ = y = None
+    gamma_prev = alpha_prev = 0.0
+    gamma = gamma0
+    gammas = [gamma0]
+    for it in range(m):
+        w = (Ah @ r.astype(np.float64)).astype(np.float32)
+        delta = float(np.dot(w.astype(np.float64), r.astype(np.float64)))
+        if it == 0:
+            alpha, beta = gamma / delta, 0.0
+        else:
+            beta = gamma / gamma_prev
+            alpha = gamma / (delta - beta * gamma / alpha_prev)
+        a, bb = np.float32(alpha), np.float32(beta)
+        if it == 0:
+            p, sp_, y = r.copy(), w.copy(), (a * r).astype(np.float32)
+        else:
+            p = _f32_fma(bb, p, r)
+            sp_ = _f32_fma(bb, sp_, w)
+            y = _f32_fma(a, p, y)
+        r = _f32_fma(-a, sp_, r)
+        gamma_prev, alpha_prev = gamma, alpha
+        gamma = float(np.dot(r.astype(np.float64), r.astype(np.float64)))
+        gammas.append(gamma)
+    return s * y.astype(np.float64), gammas
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 8, 33])
+def test_ir_scaled_inner_increment(ctx, problems, m):
+    """The increment one outer step adds, after exactly m inner iterations
+    (odd and even m; 33 crosses a replay boundary), against the numpy
+    restatement of the inner solve.  The device adds alpha p to y two
+    iterations at a time and leaves the last term of an odd count to the
+    outer step, so an error there would show here, where the refinement's
+    final residual would hide it (the outer loop corrects any increment).
+    The inner solve is stopped once by its maxIter and once by a tolerance
+    chosen between the sums of iterations m - 1 and m; the latter is solved
+    twice on one refinement object, and the second solve, predicted from
+    the first, queues its outer step behind the inner solve's last replay
+    (the increment is then added there)."""
+    pr = problems(1, 32)
+    want, gammas = _scaled_inner_cg(pr.A, pr.b, m)
+    outer = fx.params(maxIter=1, relTol=1e-14, absTol=0.0)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    # stopped by maxIter
+    ir = fx.IR(ctx, A, 1.0, outer, fx.params(maxIter=m, relTol=1e-30, absTol=0.0))
+    try:
+        x = fx.Vec(ctx, pr.A.nrow)
+        _, inner_its = ir.Solve(dev(ctx, pr.b), x)
+        assert inner_its == m
+        assert rel_err(x.numpy(), want) < 1e-5, (m, rel_err(x.numpy(), want))
+    finally:
+        ir.close()
+    # stopped by the tolerance at iteration m: the first gamma below thr
+    lo, hi = gammas[m], min(gammas[:m])
+    assert lo < 0.95 * hi, (m, lo, hi)  # a margin far above the f32 rounding differences
+    rel_tol = float(np.sqrt(np.sqrt(lo * hi) / gammas[0]))
+    ir = fx.IR(ctx, A, 1.0, outer, fx.params(maxIter=10 * m + 10, relTol=rel_tol, absTol=0.0))
+    try:
+        for _ in range(2):
+            x = fx.Vec(ctx, pr.A.nrow)
+            _, inner_its = ir.Solve(dev(ctx, pr.b), x)
+            assert inner_its == m
+            assert rel_err(x.numpy(), want) < 1e-5, (m, rel_err(x.numpy(), want))
+    finally:
+        ir.close()
+
+
 @pytest.mark.parametrize("cid,scale", [(3, 128), (1, 64)])
 def test_ir_gmres_inner(ctx, oracle, problems, cid, scale):
     """FP32-inner GMRES (f64 Hessenberg / Givens) with FP64 refinement, on the
