@@ -202,6 +202,41 @@ def test_jacobi_tolerance_repeat(ctx, oracle, problems, relTol):
     assert out[1][0] == out[0][0] and np.array_equal(out[1][1], out[0][1])
 
 
+@pytest.mark.parametrize("relTol", [1e-3, 1e-4])
+def test_jacobi_tolerance_at_max_iterations(ctx, oracle, problems, relTol):
+    """A tolerance stop after K sweeps with maxIter = K - 1, K and K + 1
+    (src/Iter.cxx:69-99: at maxIter = K the loop ends by its count with the
+    same iterate; at K + 1 the sweep that decides the last allowed one finds
+    convergence, and only its deciding workgroup runs): the counts, x bit
+    for bit and the final residual against the unlimited solve and the
+    oracle's fixed-count sweeps."""
+    pr = problems(2, 32)
+    M = Matrix.of(pr.A)
+    A = fx.Csr.from_problem(ctx, pr.A)
+    P = fx.Precond(ctx, "jacobi", A, 2 / 3, 1)
+
+    def run(max_it, tol):
+        s = fx.Solver(ctx, fx.JACOBI)
+        s.Setup(A, P, fx.params(maxIter=max_it, relTol=tol, absTol=0.0))
+        x = dev(ctx, np.zeros(M.nrow))
+        res = s.Solve(dev(ctx, pr.b), x)
+        assert s.last_path == "fused"
+        return res, x.numpy()
+
+    full, x_full = run(2000, relTol)
+    K = full.numIter
+    assert 2 < K < 2000
+    for max_it, want_it in [(K - 1, K - 1), (K, K), (K + 1, K)]:
+        res, x = run(max_it, relTol)
+        assert res.numIter == want_it, (max_it, res.numIter, K)
+        o = oracle.solve(JACOBI, M, pr.b, np.zeros(M.nrow), weight=2 / 3, maxIter=want_it,
+                         relTol=1e-30, absTol=0.0, hist_cap=2100)
+        assert np.array_equal(x, o["x"]), max_it
+        assert abs(res.norm2 - o["norm2"]) <= 1e-13 * o["norm2"], max_it
+        if want_it == K:
+            assert np.array_equal(x, x_full)
+
+
 def test_jacobi_converges_unit(ctx):
     """UnitTestsJacobi 3x3 (utest/src/UnitTestsJacobi.cxx:85-157)."""
     class PR:
