@@ -671,8 +671,9 @@ __global__ void k_add_f32_to_f64(double* __restrict__ x, const float* __restrict
 struct IrState {
   double gamma0;      // (r0, r0) of the scaled residual
   double thr;         // stop when gamma < thr: max(relTol^2 gamma0, absTol^2)
-  double gamma_prev;  // gamma and alpha of the last update
+  double gamma_prev;  // gamma, alpha and beta of the last update
   double alpha_prev;
+  double beta_prev;
   double gamma, delta;  // the last iteration's sums
   uint32_t numIter, maxIter;
   int status, reason;
@@ -833,15 +834,16 @@ constexpr int kIrPerCu = FX_IR_PER_CU;
 
 // One pass of K2 over packs c + i * stride, i < kIrPacks (all loaded
 // before use; packs at or past nc are clamped to pack 0 and dropped).
+// odd: an odd iteration, which also reads p, y and the previous
+// iteration's r (ro); see ir_elem.
 struct IrPass {
-  Pack16<float> r[kIrPacks], w[kIrPacks], p[kIrPacks], s[kIrPacks], y[kIrPacks];
+  Pack16<float> r[kIrPacks], ro[kIrPacks], w[kIrPacks], p[kIrPacks], s[kIrPacks], y[kIrPacks];
   uint32_t c[kIrPacks];
   bool ok[kIrPacks];
-  // ld_y: y is read only by the iterations that update it (see ir_elem)
-  __device__ __forceinline__ void load(const float* __restrict__ rv, const float* __restrict__ wv,
-                                       const float* __restrict__ pv, const float* __restrict__ sv,
-                                       const float* __restrict__ yv, uint32_t c0, uint32_t stride,
-                                       uint32_t nc, bool ld_y) {
+  __device__ __forceinline__ void load(const float* __restrict__ rv, const float* __restrict__ rov,
+                                       const float* __restrict__ wv, const float* __restrict__ pv,
+                                       const float* __restrict__ sv, const float* __restrict__ yv,
+                                       uint32_t c0, uint32_t stride, uint32_t nc, bool odd) {
 #pragma unroll
     for (int i = 0; i < kIrPacks; ++i) {
       const uint32_t ci = c0 + i * stride;
@@ -849,36 +851,47 @@ struct IrPass {
       c[i] = ok[i] ? ci : 0u;
       r[i] = ld16(rv, c[i]);
       w[i] = ld16(wv, c[i]);
-      p[i] = ld16(pv, c[i]);
       s[i] = ld16(sv, c[i]);
-      if (ld_y) y[i] = ld16(yv, c[i]);
+      if (odd) {
+        ro[i] = ld16(rov, c[i]);
+        p[i] = ld16(pv, c[i]);
+        y[i] = ld16(yv, c[i]);
+      }
     }
   }
 };
 
-// The updates of one element in iteration it; first (it == 0): the
-// iteration that starts the recurrences (p = r, s = w; the old p and s are
-// not used).  y += alpha p is applied two iterations at a time, so that y
-// is read and written every other iteration: an odd iteration (yup) forms
-// y = fma(a, p, fma(ap, p_old, y)) from the previous iteration's alpha (ap)
-// and p, which are the same two roundings, in the same order, as updating y
-// in each; at it == 1 (ypair0) the first term is ap * p_old, as iteration
-// 0's y = alpha p.  After an odd number of iterations the last one's term
-// is still to be added (k_ir_add_scaled does it).
-__device__ __forceinline__ void ir_elem(bool first, bool yup, bool ypair0, float a, float b,
-                                        float ap, float& ri, float wi, float& pi, float& si,
-                                        float& yi, double& acc) {
-  const float po = pi;
-  pi = first ? ri : fma(b, pi, ri);
+// The updates of one element in iteration it: s and r in every iteration
+// (first, it == 0: s = w).  p serves only y += alpha p, so both are formed
+// two iterations at a time: an odd iteration (odd) forms
+//   p_{it-1} = fma(bp, p_{it-2}, r_{it-1})     (r_0 when it == 1: pair0)
+//   p_it     = fma(b, p_{it-1}, r_it)
+//   y        = fma(a, p_it, fma(ap, p_{it-1}, y))   (ap * p_0 when it == 1)
+// from the previous iteration's r (ro, kept in the other r buffer), alpha
+// (ap) and beta (bp): the same roundings, in the same order, as forming p
+// and y in each iteration.  Even iterations read and write neither.  After
+// an odd number of iterations the last one's p and alpha p are still to be
+// formed (k_ir_add_scaled does it).
+__device__ __forceinline__ void ir_elem(bool first, bool odd, bool pair0, float a, float b,
+                                        float ap, float bp, float& ri, float roi, float wi,
+                                        float& pi, float& si, float& yi, double& acc) {
+  if (odd) {
+    const float pm = pair0 ? roi : fma(bp, pi, roi);
+    pi = fma(b, pm, ri);
+    yi = fma(a, pi, pair0 ? ap * pm : fma(ap, pm, yi));
+  }
   si = first ? wi : fma(b, si, wi);
-  if (yup) yi = fma(a, pi, ypair0 ? ap * po : fma(ap, po, yi));
   ri = fma(-a, si, ri);
   acc = fma((double)ri, (double)ri, acc);
 }
 
+// r alternates between two buffers: iteration it reads r_it from ra (even
+// it) or rb (odd it) and writes r_{it+1} to the other; an odd iteration
+// also reads r_{it-1} from ra.
 __global__ __launch_bounds__(kWideThreads) void k_ir_update(
-    float* __restrict__ r, const float* __restrict__ w, float* __restrict__ p,
-    float* __restrict__ s, float* __restrict__ y, uint32_t n, IrState* st_cur, IrState* st_next,
+    float* __restrict__ ra, float* __restrict__ rb, const float* __restrict__ w,
+    float* __restrict__ p, float* __restrict__ s, float* __restrict__ y, uint32_t n,
+    IrState* st_cur, IrState* st_next,
     IrState* hout, const double* __restrict__ p1, unsigned g1, const double* __restrict__ pg_in,
     double* __restrict__ pg_out) {
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -893,9 +906,11 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
   const uint32_t nc = n / W, stride = gridDim.x * kWideThreads;
   const uint32_t cfirst = blockIdx.x * kWideThreads + threadIdx.x;
   const uint32_t it = st_cur->numIter;
-  const bool yup = (it & 1u) != 0;  // this iteration updates y (for it - 1 and it)
+  const bool odd = (it & 1u) != 0;  // this iteration forms p and y (for it - 1 and it)
+  const float* r = odd ? rb : ra;   // r_it
+  float* rn = odd ? ra : rb;        // r_{it+1}; over r_{it-1} (ra) in an odd iteration
   IrPass pass;  // the first pass is loaded before the scalars are known
-  pass.load(r, w, p, s, y, cfirst, stride, nc, yup);
+  pass.load(r, ra, w, p, s, y, cfirst, stride, nc, odd);
   double red[2];
   {
     double d1[1], d2[1] = {0.0};
@@ -936,9 +951,10 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
     }
     return;
   }
-  const bool first = it == 0, ypair0 = it == 1;
+  const bool first = it == 0, pair0 = it == 1;
   const float a = (float)alpha, b = (float)beta;
-  const float ap = (float)st_cur->alpha_prev;  // iteration it - 1's (read when yup)
+  // iteration it - 1's alpha and beta (used when odd)
+  const float ap = (float)st_cur->alpha_prev, bp = (float)st_cur->beta_prev;
   double acc = 0.0;
   for (uint32_t c = cfirst;;) {
 #pragma unroll
@@ -946,28 +962,33 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
       double ai = 0.0;
 #pragma unroll
       for (int k = 0; k < W; ++k)
-        ir_elem(first, yup, ypair0, a, b, ap, pass.r[i].v[k], pass.w[i].v[k], pass.p[i].v[k],
-                pass.s[i].v[k], pass.y[i].v[k], ai);
+        ir_elem(first, odd, pair0, a, b, ap, bp, pass.r[i].v[k], pass.ro[i].v[k],
+                pass.w[i].v[k], pass.p[i].v[k], pass.s[i].v[k], pass.y[i].v[k], ai);
       if (pass.ok[i]) {
-        st16(r, pass.c[i], pass.r[i]);
-        st16(p, pass.c[i], pass.p[i]);
+        st16(rn, pass.c[i], pass.r[i]);
         st16(s, pass.c[i], pass.s[i]);
-        if (yup) st16(y, pass.c[i], pass.y[i]);
+        if (odd) {
+          st16(p, pass.c[i], pass.p[i]);
+          st16(y, pass.c[i], pass.y[i]);
+        }
         acc += ai;
       }
     }
     c += kIrPacks * stride;
     if (c >= nc) break;
-    pass.load(r, w, p, s, y, c, stride, nc, yup);
+    pass.load(r, ra, w, p, s, y, c, stride, nc, odd);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nc * W) {
     const uint32_t i = nc * W + threadIdx.x;
-    float ri = r[i], pi = p[i], si = s[i], yi = yup ? y[i] : 0.0f;
-    ir_elem(first, yup, ypair0, a, b, ap, ri, w[i], pi, si, yi, acc);
-    r[i] = ri;
-    p[i] = pi;
+    float ri = r[i], si = s[i];
+    float roi = odd ? ra[i] : 0.0f, pi = odd ? p[i] : 0.0f, yi = odd ? y[i] : 0.0f;
+    ir_elem(first, odd, pair0, a, b, ap, bp, ri, roi, w[i], pi, si, yi, acc);
+    rn[i] = ri;
     s[i] = si;
-    if (yup) y[i] = yi;
+    if (odd) {
+      p[i] = pi;
+      y[i] = yi;
+    }
   }
   double v[1] = {acc};
   block_partials_n<1, 0u, kWideThreads>(v, pg_out);
@@ -975,6 +996,7 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
     copy_state(st_next, st_cur);
     st_next->gamma_prev = gamma;
     st_next->alpha_prev = alpha;
+    st_next->beta_prev = beta;
     st_next->gamma = gamma;
     st_next->delta = delta;
     st_next->numIter = it + 1;
@@ -985,21 +1007,26 @@ __global__ __launch_bounds__(kWideThreads) void k_ir_update(
 // x += s_i d_i: the scaled correction back in the unscaled unknowns.
 // x += S y; with cs, only after an inner solve that ran at least one
 // iteration to its end (y holds that solve's increment).  fin: the inner
-// solve's final state; after an odd number m of iterations y still lacks
-// the last one's alpha p (k_ir_update adds two iterations' terms at a
-// time), which is added here with the same rounding (alpha p alone when
-// m == 1).
+// solve's final state.  After an odd number m of iterations (k_ir_update
+// forms p and y two iterations at a time) the last iteration's p and
+// alpha p are still to be formed, as ir_elem would: p_{m-1} =
+// fma(beta, p_{m-2}, r_{m-1}) with r_{m-1} in ra (r_0 when m == 1), then
+// y = fma(alpha, p_{m-1}, y) (alpha p_0 when m == 1).
 __global__ void k_ir_add_scaled(double* __restrict__ x, const double* __restrict__ sc,
                                 const float* __restrict__ y, const float* __restrict__ p,
-                                uint32_t n, const IrState* cs, const IrState* fin) {
+                                const float* __restrict__ ra, uint32_t n, const IrState* cs,
+                                const IrState* fin) {
   if (cs && (ir_inner_open(cs) || cs->numIter == 0)) return;
   const uint32_t m = fin->numIter;
   const bool pend = (m & 1u) != 0;
-  const float ap = (float)fin->alpha_prev;
+  const float ap = (float)fin->alpha_prev, bp = (float)fin->beta_prev;
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
     float yi = m == 1 ? 0.0f : y[i];
-    if (pend) yi = m == 1 ? ap * p[i] : fma(ap, p[i], yi);
+    if (pend) {
+      const float pm = m == 1 ? ra[i] : fma(bp, p[i], ra[i]);
+      yi = m == 1 ? ap * pm : fma(ap, pm, yi);
+    }
     x[i] = fma(sc[i], (double)yi, x[i]);
   }
 }
@@ -1465,9 +1492,10 @@ int jacobi_run(fx_solver S, FastWs& W, const double* b_user, double* x_user, fx_
 }
 
 // The refinement's scaled inner solve (k_ir_update), replayed in graphs of
-// kChunk iterations on an f32 workspace: r = W.r, w = W.q, p = W.p,
-// s = W.x2, y = W.x.  It starts from the state FinIrInit wrote to state
-// buffer 0 and r from EpiResidScaled.
+// kChunk iterations on an f32 workspace: r = W.r (even iterations' r) and
+// W.s (odd iterations'), w = W.q, p = W.p, s = W.x2, y = W.x.  It starts
+// from the state FinIrInit wrote to state buffer 0 and r from
+// EpiResidScaled (in W.r).
 struct IrInner {
   FastWs& W;
   fx_csr As;  // A's structure with the scaled f32 values
@@ -1479,14 +1507,16 @@ struct IrInner {
   double* pg(int c) const { return W.partials + W.part2_off + (size_t)(c & 1) * kMaxRedBlocks; }
 
   int launch_iteration(int c, IrState* hout) {
-    float* r = (float*)W.r;
+    float* ra = (float*)W.r;
+    float* rb = (float*)W.s;
+    float* r = (c & 1) ? rb : ra;  // replays start at even iterations
     float* w = (float*)W.q;
     FX_TRY((launch_spmv<float, float, ROW_AX, 1, 0u>(As, st, XLoad<float>{r}, BNone{},
                                                      EpiIrW{w, r, dstate(c)}, W.partials,
                                                      nullptr, nullptr, false)));
     const unsigned g1 = spmv_grid_of<float>(As, W.ctx->sms);
     const unsigned g = wide_blocks(W.n, Pack16<float>::W, W.ctx->sms, kIrPacks, kIrPerCu);
-    hipLaunchKernelGGL(k_ir_update, dim3(g), dim3(kWideThreads), 0, st, r, (const float*)w,
+    hipLaunchKernelGGL(k_ir_update, dim3(g), dim3(kWideThreads), 0, st, ra, rb, (const float*)w,
                        (float*)W.p, (float*)W.x2, (float*)W.x, W.n, dstate(c), dstate(c + 1),
                        hout, (const double*)W.partials, g1, (const double*)pg(c), pg(c + 1));
     FX_HIP(hipGetLastError());
@@ -1763,6 +1793,11 @@ int fx_ir_create(fx_ctx ctx, int inner_type, fx_csr A, double weight, const fx_p
     const size_t sb = sizeof(CgState) > sizeof(IrState) ? sizeof(CgState) : sizeof(IrState);
     rc = ws_alloc(*ir->W, ctx, FX_F32, n, inner->maxIter, sb, spmv_grid_of<float>(A, ctx->sms));
     if (rc) return fail(rc);
+    // the scaled inner solve's second r buffer (odd iterations' r)
+    if (ir->scaled && hipMalloc(&ir->W->s, sizeof(float) * pack_round(n))) {
+      set_error("IR workspace allocation failed");
+      return fail(FX_ERROR_ALLOC_MEM);
+    }
   }
   *out = ir;
   return 0;
@@ -1792,7 +1827,8 @@ int fx_ir_solve(fx_ir ir, fx_vec b, fx_vec x, fx_result* result, uint32_t* inner
     if (add) {
       hipLaunchKernelGGL(k_ir_add_scaled, dim3(stream_blocks(n)), dim3(kThreads), 0, st,
                          (double*)x->d, (const double*)ir->sc, (const float*)ir->W->x,
-                         (const float*)ir->W->p, n, cs, (const IrState*)ir->W->state);
+                         (const float*)ir->W->p, (const float*)ir->W->r, n, cs,
+                         (const IrState*)ir->W->state);
       FX_HIP(hipGetLastError());
     }
     if (ir->bo)
